@@ -1,0 +1,21 @@
+"""MI355X-native batched KPP Rosenbrock Rodas3 gas-phase chemistry solver.
+
+The product is the C-ABI library ``lib/libgcchemgpu.so`` (declarations in
+``include/gcchemgpu.h``), a drop-in for the library GEOS-Chem's fullchem
+driver binds through ISO_C_BINDING.  This Python package is the host-side
+mirror used by the standalone driver, the tests and the gradient path:
+
+* ``mechanism``  mechanism IR loading and counts (flops, bytes)
+* ``lib``        ctypes binding of the C ABI (fails loudly without the library)
+* ``solver``     device-resident batch solve on torch tensors
+* ``inputs``     seeded synthetic cells for the benchmark configurations
+* ``standalone`` batch driver (the reference's kpp_batch modes)
+* ``autotune``   launch-parameter sweep of the persistent kernel
+
+The directory name is not a Python identifier; it is imported as
+``gcchem_amd`` (see ``load_package`` in the repository's entry points).
+"""
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent
+REPO_DIR = PKG_DIR.parent

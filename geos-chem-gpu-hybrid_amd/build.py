@@ -1,0 +1,136 @@
+"""Build the gfx950 solver libraries (one per mechanism).
+
+For each mechanism IR under ``mechanisms/``:
+
+1. ``codegen/gen_hip.py`` writes ``_build/<mech>/mech_gen.inc`` (chunked
+   straight-line FP64 HIP for Fun, Jac_SP+prep, KppDecomp, KppSolve,
+   KppSolveTR);
+2. ``csrc/gc_rodas3.hip`` is compiled once per register-budget variant
+   (``-DGC_WPE=1``: 512 VGPR/lane, ``-DGC_WPE=2``: 256 VGPR/lane) and
+   ``csrc/gc_api.hip`` once, in parallel;
+3. the objects are linked into ``lib/libgcchemgpu_<mech>.so``; the fullchem
+   library is also installed as ``lib/libgcchemgpu.so``, the drop-in name the
+   model links (src/GCClassic/CMakeLists.txt:204-228 links ``GC_GPU_LIB``).
+
+Every HIP file is compiled with ``-ffp-contract=off`` (no FMA contraction:
+the reference CPU build evaluates every product and sum separately) and with
+packed-FP32 instruction selection disabled.
+"""
+from __future__ import annotations
+
+import json
+import os
+import shutil
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "lib"
+MECH_DIR = PKG / "mechanisms"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+VARIANTS = (1, 2)
+
+
+def lib_path(mech: str = "fullchem") -> Path:
+    return LIB / f"libgcchemgpu_{mech}.so"
+
+
+def _stale(out: Path, deps) -> bool:
+    if not out.exists():
+        return True
+    t = out.stat().st_mtime
+    return any(Path(d).stat().st_mtime > t for d in deps)
+
+
+def _run_parallel(jobs, max_jobs):
+    """jobs: list of (argv, label). Raises on the first failure."""
+    running = []
+    pending = list(jobs)
+    while pending or running:
+        while pending and len(running) < max_jobs:
+            argv, label = pending.pop(0)
+            p = subprocess.Popen(argv, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            running.append((p, label, time.time()))
+        for item in list(running):
+            p, label, t0 = item
+            if p.poll() is not None:
+                out = p.stdout.read()
+                running.remove(item)
+                if p.returncode != 0:
+                    for q, _, _ in running:
+                        q.kill()
+                    raise RuntimeError(f"{label} failed:\n{out[-4000:]}")
+                print(f"  built {label} ({time.time() - t0:.0f} s)", flush=True)
+        time.sleep(0.2)
+
+
+def build_mechanism(mech: str, force: bool = False, jobs: int | None = None) -> Path:
+    sys.path.insert(0, str(PKG / "codegen"))
+    import gen_hip  # noqa: E402
+
+    ir_path = MECH_DIR / f"{mech}.json"
+    bdir = BUILD / mech
+    bdir.mkdir(parents=True, exist_ok=True)
+    LIB.mkdir(exist_ok=True)
+    inc = bdir / "mech_gen.inc"
+    gen_deps = [ir_path, PKG / "codegen" / "gen_hip.py"]
+    if force or _stale(inc, gen_deps):
+        ir = json.loads(ir_path.read_text())
+        text = gen_hip.generate(ir)
+        if not inc.exists() or inc.read_text() != text:
+            inc.write_text(text)
+    common = [CSRC / "gc_math.h", CSRC / "gc_rodas3_kernels.h"]
+    todo, objs = [], []
+    for w in VARIANTS:
+        obj = bdir / f"rodas3_w{w}.o"
+        objs.append(obj)
+        deps = [CSRC / "gc_rodas3.hip", inc] + common
+        if force or _stale(obj, deps):
+            todo.append(([HIPCC, *HIP_FLAGS, f"-DGC_WPE={w}", f"-I{CSRC}", f"-I{bdir}",
+                          "-c", str(CSRC / "gc_rodas3.hip"), "-o", str(obj)],
+                         f"{mech}/rodas3_w{w}.o"))
+    obj = bdir / "api.o"
+    objs.append(obj)
+    deps = [CSRC / "gc_api.hip", PKG.parent / "include" / "gcchemgpu.h"] + common
+    if force or _stale(obj, deps):
+        todo.append(([HIPCC, *HIP_FLAGS, f"-I{CSRC}", "-c", str(CSRC / "gc_api.hip"),
+                      "-o", str(obj)], f"{mech}/api.o"))
+    if todo:
+        _run_parallel(todo, jobs or max(1, min(len(todo), os.cpu_count() or 1)))
+    out = lib_path(mech)
+    if force or _stale(out, objs):
+        tmp = out.with_suffix(".so.tmp")
+        subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
+                        *map(str, objs)], check=True)
+        os.replace(tmp, out)
+    if mech == "fullchem":
+        drop_in = LIB / "libgcchemgpu.so"
+        if force or _stale(drop_in, [out]):
+            shutil.copyfile(out, drop_in)
+    return out
+
+
+def available_mechanisms():
+    return sorted(p.stem for p in MECH_DIR.glob("*.json"))
+
+
+def build_all(force: bool = False, mechs=None) -> list[Path]:
+    return [build_mechanism(m, force=force) for m in (mechs or available_mechanisms())]
+
+
+if __name__ == "__main__":
+    force = "--force" in sys.argv
+    mechs = [a for a in sys.argv[1:] if not a.startswith("--")] or None
+    for p in build_all(force=force, mechs=mechs):
+        print(p)

@@ -1,0 +1,494 @@
+// gc_api.hip -- C ABI of libgcchemgpu.so (declarations: include/gcchemgpu.h).
+//
+// Host side of the batched Rodas3 solve: device selection and query, per-device
+// buffers and stream, host<->device staging for the reference entry points, the
+// launch-order validation, the plain / persistent / chunked launch forms, and
+// launch tuning.  The library does no I/O (as gpulib/src/gc_gpu_batch_api.F90:
+// 11-15 requires of the reference): errors are reported through IERR, return
+// codes and gc_amd_last_error().
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gcchemgpu.h"
+#include "gc_rodas3_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const char* where, hipError_t e) {
+  g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+}
+void set_error(const char* msg) { g_last_error = msg; }
+
+GcMechDims mech_dims() {
+  static GcMechDims d = [] {
+    GcMechDims x;
+    gc_mech_dims_w1(&x);
+    return x;
+  }();
+  return d;
+}
+
+long ws_per_slot() {
+  const GcMechDims d = mech_dims();
+  return GC_WS_DOUBLES_PER_SLOT(d.nvar, d.nfix, d.nreact, d.nnz);
+}
+
+// ---- launch tuning (process-wide) ------------------------------------------
+struct Tuning {
+  int block = 256;
+  int blocks_per_cu = 1;
+  int cells_per_wave = 64;
+  int grain = 64;
+  int regs = 1;
+};
+std::mutex g_tune_mu;
+Tuning g_tune;
+bool g_tune_env_read = false;
+
+Tuning current_tuning() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (!g_tune_env_read) {
+    g_tune_env_read = true;
+    // GC_AMD_TUNE="block,blocks_per_cu,cells_per_wave,grain,regs"
+    if (const char* s = getenv("GC_AMD_TUNE")) {
+      int v[5] = {0, 0, 0, 0, 0};
+      if (sscanf(s, "%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4]) == 5) {
+        if (v[0] >= 64 && v[0] <= 256 && v[0] % 64 == 0) g_tune.block = v[0];
+        if (v[1] >= 1 && v[1] <= 32) g_tune.blocks_per_cu = v[1];
+        if (v[2] >= 1 && v[2] <= 64) g_tune.cells_per_wave = v[2];
+        if (v[3] >= 1 && v[3] <= 4096) g_tune.grain = v[3];
+        if (v[4] == 1 || v[4] == 2) g_tune.regs = v[4];
+      }
+    }
+  }
+  return g_tune;
+}
+
+// ---- per-device context -----------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    else p = nullptr;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+struct DevCtx {
+  std::mutex mu;
+  bool init = false;
+  hipStream_t stream = nullptr;
+  int n_cu = 0;
+  DevBuf ws, queue, flag;
+  // staging for the host-array entry points
+  DevBuf C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+constexpr int kMaxDev = 64;
+DevCtx g_ctx[kMaxDev];
+
+hipError_t ctx_init(DevCtx& c, int dev) {
+  if (c.init) return hipSuccess;
+  hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return e;
+  if ((e = hipEventCreate(&c.ev0)) != hipSuccess) return e;
+  if ((e = hipEventCreate(&c.ev1)) != hipSuccess) return e;
+  hipDeviceProp_t prop;
+  if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) return e;
+  c.n_cu = prop.multiProcessorCount;
+  c.init = true;
+  return hipSuccess;
+}
+
+DevCtx* current_ctx(int* dev_out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return nullptr;
+  if (dev_out) *dev_out = dev;
+  return &g_ctx[dev];
+}
+
+// ---- small utility kernels ---------------------------------------------------
+__global__ void k_check_perm(const int* perm, int n, int* bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int p = perm[i];
+    if (p < 1 || p > n) atomicOr(bad, 1);
+  }
+}
+
+__global__ void k_zero_u32(unsigned* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// ---- the device solve ----------------------------------------------------------
+// Launches one chunk (a contiguous range of the launch order) and returns the
+// runtime status of the launch.
+hipError_t launch_chunk(DevCtx& c, int kernel, const Tuning& t, const GcBatchArgs& b,
+                        unsigned S, hipStream_t st) {
+  double* ws = (double*)c.ws.p;
+  if (kernel == GC_AMD_KERNEL_PLAIN) {
+    return gc_launch_plain_w1(&b, ws, S, 128, st);
+  }
+  unsigned* q = (unsigned*)c.queue.p;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, q, 1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (t.regs == 2)
+    return gc_launch_persistent_w2(&b, ws, S, q, t.block, t.cells_per_wave, t.grain, st);
+  return gc_launch_persistent_w1(&b, ws, S, q, t.block, t.cells_per_wave, t.grain, st);
+}
+
+// Number of slots (lanes with workspace) for a launch of n cells.
+unsigned slots_for(DevCtx& c, int kernel, const Tuning& t, int n) {
+  if (kernel == GC_AMD_KERNEL_PLAIN) return ((unsigned)n + 63u) & ~63u;
+  // persistent: one resident grid; no more waves than there are cells to hand out
+  const unsigned grid_cap = (unsigned)(c.n_cu * t.blocks_per_cu);
+  const unsigned cells_per_block = (unsigned)(t.block / 64 * t.cells_per_wave);
+  unsigned need = ((unsigned)n + cells_per_block - 1u) / cells_per_block;
+  if (need < 1u) need = 1u;
+  const unsigned grid = need < grid_cap ? need : grid_cap;
+  return grid * (unsigned)t.block;
+}
+
+int device_solve(DevCtx& c, int kernel, int chunk, const GcBatchArgs& b, hipStream_t st,
+                 double* tKernel) {
+  if (tKernel) *tKernel = 0.0;
+  if (b.nCell < 0 || (kernel != GC_AMD_KERNEL_PLAIN && kernel != GC_AMD_KERNEL_PERSISTENT) ||
+      chunk < 0) {
+    set_error("gc_amd_integrate_batch_device: bad argument");
+    return -1;
+  }
+  if (b.nCell == 0) return 0;
+  const Tuning t = current_tuning();
+  hipError_t e;
+
+  // launch order must be a map into 1..nCell (a bad entry would address
+  // outside the caller's arrays)
+  if ((e = c.flag.ensure(sizeof(int))) != hipSuccess) { set_error("hipMalloc(flag)", e); return -2; }
+  if ((e = hipMemsetAsync(c.flag.p, 0, sizeof(int), st)) != hipSuccess) { set_error("hipMemsetAsync", e); return -3; }
+  hipLaunchKernelGGL(k_check_perm, dim3((b.nCell + 255) / 256), dim3(256), 0, st, b.Perm, b.nCell,
+                     (int*)c.flag.p);
+  int bad = 0;
+  if ((e = hipMemcpyAsync(&bad, c.flag.p, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess) {
+    set_error("perm check", e);
+    return -3;
+  }
+  if (bad) {
+    set_error("gc_amd_integrate_batch_device: Perm entry outside 1..nCell");
+    return -1;
+  }
+
+  const int span = (chunk > 0 && chunk < b.nCell) ? chunk : b.nCell;
+  const unsigned S = slots_for(c, kernel, t, span);
+  if ((e = c.ws.ensure((size_t)S * (size_t)ws_per_slot() * sizeof(double))) != hipSuccess) {
+    set_error("hipMalloc(workspace)", e);
+    return -2;
+  }
+  if ((e = c.queue.ensure(256)) != hipSuccess) { set_error("hipMalloc(queue)", e); return -2; }
+
+  if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+  if (span == b.nCell) {
+    e = launch_chunk(c, kernel, t, b, S, st);
+    if (e != hipSuccess) { set_error("launch", e); return -3; }
+  } else {
+    // chunk loop over the launch order, captured once and replayed as a graph
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+      set_error("hipStreamBeginCapture", e);
+      return -3;
+    }
+    hipError_t le = hipSuccess;
+    for (int c0 = 0; c0 < b.nCell && le == hipSuccess; c0 += span) {
+      GcBatchArgs sub = b;
+      sub.Perm = b.Perm + c0;
+      sub.nCell = (b.nCell - c0 < span) ? (b.nCell - c0) : span;
+      le = launch_chunk(c, kernel, t, sub, S, st);
+    }
+    e = hipStreamEndCapture(st, &g);
+    if (le != hipSuccess || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      set_error("chunk capture", le != hipSuccess ? le : e);
+      return -3;
+    }
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(ge, st);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { set_error("chunk graph", e); return -3; }
+  }
+  if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) { set_error("solve", e); return -3; }
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess && tKernel) *tKernel = ms * 1e-3;
+  return 0;
+}
+
+// env helpers mirroring gpulib/src/gc_gpu_batch_api.F90:231-256
+int env_kernel() {
+  const char* s = getenv("GC_GPU_KERNEL");
+  if (s && (strncmp(s, "plain", 5) == 0 || strncmp(s, "PLAIN", 5) == 0)) return GC_AMD_KERNEL_PLAIN;
+  return GC_AMD_KERNEL_PERSISTENT;
+}
+int env_chunk(int nCell) {
+  const char* s = getenv("GC_GPU_WORKSPACE");
+  if (!s || !(strncmp(s, "global", 6) == 0 || strncmp(s, "GLOBAL", 6) == 0)) return 0;
+  int v = 32768;
+  if (const char* cs = getenv("GC_GPU_CHUNK")) {
+    int val = 0;
+    bool ok = *cs != 0;
+    for (const char* p = cs; *p; ++p) {
+      if (*p < '0' || *p > '9') { ok = false; break; }
+      val = val * 10 + (*p - '0');
+    }
+    if (ok && val > 0) v = val;
+  }
+  if (v > nCell) v = nCell;
+  if (v < 1024) v = 1024;
+  return v;
+}
+
+void fill_ierr(int* IERR, int n, int code) {
+  for (int i = 0; i < n; ++i) IERR[i] = code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gc_gpu_set_device(int dev) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || dev < 0 || dev >= n) return -1;
+  if (hipSetDevice(dev) != hipSuccess) return -1;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) return -1;
+  return cur;
+}
+
+void gc_gpu_info(int* nDev, char* devName, int nameLen) {
+  *nDev = 0;
+  if (nameLen > 0) devName[0] = '\0';
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return;
+  *nDev = n;
+  if (n > 0 && nameLen > 1) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess) {
+      int m = (int)strnlen(prop.name, sizeof(prop.name));
+      if (m == 0) m = (int)strnlen(prop.gcnArchName, sizeof(prop.gcnArchName));
+      const char* src = prop.name[0] ? prop.name : prop.gcnArchName;
+      if (m > nameLen - 1) m = nameLen - 1;
+      memcpy(devName, src, (size_t)m);
+      devName[m] = '\0';
+    }
+  }
+}
+
+void gc_gpu_integrate_batch_dev(int dev, int nCell, double Tstart, double Tend, double* C,
+                                const double* RCONST, const double* ATOL, const double* RTOL,
+                                const int* ICNTRL, const double* RCNTRL, const int* Perm,
+                                int* ISTAT, double* RSTAT, int* IERR, double* tTotal,
+                                double* tKernel) {
+  *tTotal = 0.0;
+  *tKernel = 0.0;
+  if (dev >= 0 && gc_gpu_set_device(dev) != dev) {
+    fill_ierr(IERR, nCell, -99);
+    return;
+  }
+  if (nCell <= 0) return;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < nCell; ++i) {
+    if (Perm[i] < 1 || Perm[i] > nCell) {
+      set_error("gc_gpu_integrate_batch: Perm entry outside 1..nCell");
+      fill_ierr(IERR, nCell, -98);
+      return;
+    }
+  }
+  int d = -1;
+  DevCtx* cp = current_ctx(&d);
+  if (!cp) {
+    set_error("no current device");
+    fill_ierr(IERR, nCell, -98);
+    return;
+  }
+  DevCtx& c = *cp;
+  std::lock_guard<std::mutex> lk(c.mu);
+  hipError_t e = ctx_init(c, d);
+  const GcMechDims md = mech_dims();
+  const size_t n = (size_t)nCell;
+  const size_t nspec = (size_t)(md.nvar + md.nfix);
+  struct { DevBuf* b; const void* h; size_t bytes; } in[] = {
+      {&c.C, C, n * nspec * 8}, {&c.RCONST, RCONST, n * md.nreact * 8},
+      {&c.ATOL, ATOL, (size_t)md.nvar * 8}, {&c.RTOL, RTOL, (size_t)md.nvar * 8},
+      {&c.ICNTRL, ICNTRL, n * 20 * 4}, {&c.RCNTRL, RCNTRL, n * 20 * 8}, {&c.Perm, Perm, n * 4}};
+  for (auto& x : in) {
+    if (e == hipSuccess) e = x.b->ensure(x.bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(x.b->p, x.h, x.bytes, hipMemcpyHostToDevice, c.stream);
+  }
+  if (e == hipSuccess) e = c.ISTAT.ensure(n * 8 * 4);
+  if (e == hipSuccess) e = c.RSTAT.ensure(n * 3 * 8);
+  if (e == hipSuccess) e = c.IERR.ensure(n * 4);
+  if (e != hipSuccess) {
+    set_error("staging", e);
+    fill_ierr(IERR, nCell, -98);
+    return;
+  }
+  GcBatchArgs b;
+  b.nCell = nCell;
+  b.Tstart = Tstart;
+  b.Tend = Tend;
+  b.C = (double*)c.C.p;
+  b.RCONST = (const double*)c.RCONST.p;
+  b.ATOL = (const double*)c.ATOL.p;
+  b.RTOL = (const double*)c.RTOL.p;
+  b.ICNTRL = (const int*)c.ICNTRL.p;
+  b.RCNTRL = (const double*)c.RCNTRL.p;
+  b.Perm = (const int*)c.Perm.p;
+  b.ISTAT = (int*)c.ISTAT.p;
+  b.RSTAT = (double*)c.RSTAT.p;
+  b.IERR = (int*)c.IERR.p;
+  const int kernel = env_kernel();
+  const int rc = device_solve(c, kernel, kernel == GC_AMD_KERNEL_PLAIN ? env_chunk(nCell) : 0, b,
+                              c.stream, tKernel);
+  if (rc != 0) {
+    fill_ierr(IERR, nCell, -98);
+    return;
+  }
+  struct { void* h; const DevBuf* b; size_t bytes; } out[] = {
+      {C, &c.C, n * nspec * 8}, {ISTAT, &c.ISTAT, n * 8 * 4}, {RSTAT, &c.RSTAT, n * 3 * 8},
+      {IERR, &c.IERR, n * 4}};
+  for (auto& x : out)
+    if (e == hipSuccess) e = hipMemcpyAsync(x.h, x.b->p, x.bytes, hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  if (e != hipSuccess) {
+    set_error("copy back", e);
+    fill_ierr(IERR, nCell, -98);
+    return;
+  }
+  *tTotal = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void gc_gpu_integrate_batch(int nCell, double Tstart, double Tend, double* C,
+                            const double* RCONST, const double* ATOL, const double* RTOL,
+                            const int* ICNTRL, const double* RCNTRL, const int* Perm, int* ISTAT,
+                            double* RSTAT, int* IERR, double* tTotal, double* tKernel) {
+  gc_gpu_integrate_batch_dev(-1, nCell, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm,
+                             ISTAT, RSTAT, IERR, tTotal, tKernel);
+}
+
+void gc_gpu_selftest(int* rc, int* nsteps, double* tTotal) {
+  const GcMechDims md = mech_dims();
+  std::vector<double> C(md.nvar + md.nfix, 1.0e6), RCONST(md.nreact, 0.0);
+  std::vector<double> ATOL(md.nvar, 1.0e-2), RTOL(md.nvar, 0.5e-2), RCNTRL(20, 0.0), RSTAT(3);
+  std::vector<int> ICNTRL(20, 0), ISTAT(8), IERR(1), Perm(1, 1);
+  ICNTRL[0] = 1;
+  ICNTRL[1] = 0;
+  ICNTRL[2] = 4;
+  ICNTRL[6] = 1;
+  ICNTRL[14] = -1;
+  double tK = 0.0;
+  gc_gpu_integrate_batch(1, 0.0, 1200.0, C.data(), RCONST.data(), ATOL.data(), RTOL.data(),
+                         ICNTRL.data(), RCNTRL.data(), Perm.data(), ISTAT.data(), RSTAT.data(),
+                         IERR.data(), tTotal, &tK);
+  *rc = IERR[0];
+  *nsteps = ISTAT[2];
+}
+
+int gc_amd_integrate_batch_device(int kernel, int chunk, int nCell, double Tstart, double Tend,
+                                  double* C, const double* RCONST, const double* ATOL,
+                                  const double* RTOL, const int* ICNTRL, const double* RCNTRL,
+                                  const int* Perm, int* ISTAT, double* RSTAT, int* IERR,
+                                  void* stream, double* tKernel) {
+  int d = -1;
+  DevCtx* cp = current_ctx(&d);
+  if (!cp) {
+    set_error("no current device");
+    return -3;
+  }
+  std::lock_guard<std::mutex> lk(cp->mu);
+  hipError_t e = ctx_init(*cp, d);
+  if (e != hipSuccess) {
+    set_error("device init", e);
+    return -3;
+  }
+  GcBatchArgs b{nCell, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR};
+  hipStream_t st = stream ? (hipStream_t)stream : cp->stream;
+  return device_solve(*cp, kernel, chunk, b, st, tKernel);
+}
+
+int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave, int grain, int regs) {
+  (void)current_tuning();  // apply GC_AMD_TUNE first so explicit values win
+  if ((block > 0 && (block < 64 || block > 256 || block % 64 != 0)) ||
+      (blocks_per_cu > 32) || (cells_per_wave > 64) || (grain > 4096) ||
+      (regs > 0 && regs != 1 && regs != 2))
+    return -1;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (block > 0) g_tune.block = block;
+  if (blocks_per_cu > 0) g_tune.blocks_per_cu = blocks_per_cu;
+  if (cells_per_wave > 0) g_tune.cells_per_wave = cells_per_wave;
+  if (grain > 0) g_tune.grain = grain;
+  if (regs > 0) g_tune.regs = regs;
+  return 0;
+}
+
+void gc_amd_get_tuning(int* block, int* blocks_per_cu, int* cells_per_wave, int* grain,
+                       int* regs) {
+  const Tuning t = current_tuning();
+  *block = t.block;
+  *blocks_per_cu = t.blocks_per_cu;
+  *cells_per_wave = t.cells_per_wave;
+  *grain = t.grain;
+  *regs = t.regs;
+}
+
+void gc_amd_mech_info(int* nvar, int* nfix, int* nreact, int* lu_nonzero) {
+  const GcMechDims d = mech_dims();
+  *nvar = d.nvar;
+  *nfix = d.nfix;
+  *nreact = d.nreact;
+  *lu_nonzero = d.nnz;
+}
+
+long gc_amd_workspace_doubles_per_slot(void) { return ws_per_slot(); }
+
+const char* gc_amd_last_error(void) { return g_last_error.c_str(); }
+
+void gc_amd_release(void) {
+  int d = -1;
+  DevCtx* cp = current_ctx(&d);
+  if (!cp) return;
+  std::lock_guard<std::mutex> lk(cp->mu);
+  DevBuf* bufs[] = {&cp->ws, &cp->queue, &cp->flag, &cp->C, &cp->RCONST, &cp->ATOL, &cp->RTOL,
+                    &cp->ICNTRL, &cp->RCNTRL, &cp->Perm, &cp->ISTAT, &cp->RSTAT, &cp->IERR};
+  for (DevBuf* b : bufs) b->release();
+}
+
+}  // extern "C"

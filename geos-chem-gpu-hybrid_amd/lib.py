@@ -1,0 +1,83 @@
+"""ctypes binding of the C ABI in include/gcchemgpu.h.
+
+Loads ``lib/libgcchemgpu_<mechanism>.so`` (built in-tree by ``build.py``).
+There is no fallback: if the library is missing, loading raises, so a GPU
+run can never silently take a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+from functools import lru_cache
+from pathlib import Path
+
+from . import PKG_DIR
+
+LIB_DIR = PKG_DIR / "lib"
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+IP = ctypes.POINTER(ctypes.c_int)
+DP = ctypes.POINTER(ctypes.c_double)
+
+# every symbol of include/gcchemgpu.h with its signature (restype, argtypes)
+SIGNATURES = {
+    "gc_gpu_integrate_batch": (None, [I, D, D, P, P, P, P, P, P, P, P, P, P, DP, DP]),
+    "gc_gpu_integrate_batch_dev": (None, [I, I, D, D, P, P, P, P, P, P, P, P, P, P, DP, DP]),
+    "gc_gpu_info": (None, [IP, ctypes.c_char_p, I]),
+    "gc_gpu_set_device": (I, [I]),
+    "gc_gpu_selftest": (None, [IP, IP, DP]),
+    "gc_amd_integrate_batch_device": (I, [I, I, I, D, D, P, P, P, P, P, P, P, P, P, P, P, DP]),
+    "gc_amd_set_tuning": (I, [I, I, I, I, I]),
+    "gc_amd_get_tuning": (None, [IP, IP, IP, IP, IP]),
+    "gc_amd_mech_info": (None, [IP, IP, IP, IP]),
+    "gc_amd_workspace_doubles_per_slot": (ctypes.c_long, []),
+    "gc_amd_last_error": (ctypes.c_char_p, []),
+    "gc_amd_release": (None, []),
+}
+
+KERNEL_PLAIN = 0
+KERNEL_PERSISTENT = 1
+
+
+def lib_path(mech: str = "fullchem") -> Path:
+    return LIB_DIR / f"libgcchemgpu_{mech}.so"
+
+
+@lru_cache(maxsize=None)
+def load(mech: str = "fullchem") -> ctypes.CDLL:
+    path = lib_path(mech)
+    if not path.exists():
+        raise RuntimeError(
+            f"{path} is missing: build the HIP library first "
+            f"(python geos-chem-gpu-hybrid_amd/build.py {mech})")
+    lib = ctypes.CDLL(str(path))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def last_error(lib) -> str:
+    s = lib.gc_amd_last_error()
+    return s.decode() if s else ""
+
+
+def mech_info(lib):
+    v = [ctypes.c_int() for _ in range(4)]
+    lib.gc_amd_mech_info(*(ctypes.byref(x) for x in v))
+    return tuple(x.value for x in v)
+
+
+def get_tuning(lib) -> dict:
+    v = [ctypes.c_int() for _ in range(5)]
+    lib.gc_amd_get_tuning(*(ctypes.byref(x) for x in v))
+    keys = ("block", "blocks_per_cu", "cells_per_wave", "grain", "regs")
+    return dict(zip(keys, (x.value for x in v)))
+
+
+def set_tuning(lib, block=0, blocks_per_cu=0, cells_per_wave=0, grain=0, regs=0):
+    rc = lib.gc_amd_set_tuning(block, blocks_per_cu, cells_per_wave, grain, regs)
+    if rc != 0:
+        raise ValueError(f"gc_amd_set_tuning rejected {block, blocks_per_cu, cells_per_wave, grain, regs}")

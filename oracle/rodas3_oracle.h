@@ -1,0 +1,80 @@
+/*
+ * rodas3_oracle.h -- CPU oracle for the batched KPP Rosenbrock Rodas3 solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference's CPU algorithm, used as the checker for the HIP product path and
+ * as the timed CPU baseline.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's CPU-baseline leg may load it; the product library never does.
+ *
+ * The mechanism is given as "sum programs" built from the mechanism IR
+ * (see geos-chem-gpu-hybrid_amd/codegen/kpp_ingest.py): every output value is
+ * a signed sum of left-to-right products, evaluated exactly in the order the
+ * KPP-generated Fortran evaluates it.
+ */
+#ifndef RODAS3_ORACLE_H
+#define RODAS3_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* factor kinds */
+enum { OR_LIT = 0, OR_A = 1, OR_RCT = 2, OR_V = 3, OR_F = 4, OR_B = 5 };
+
+typedef struct {
+    int nout;            /* number of outputs                              */
+    const int *out_off;  /* [nout+1] term range per output                 */
+    const int *term_sign;/* [nterm] +1 / -1                                */
+    const int *term_off; /* [nterm+1] factor range per term                */
+    const int *fac_kind; /* [nfac] OR_*                                    */
+    const int *fac_idx;  /* [nfac] 0-based index (literal table for OR_LIT)*/
+} or_sumprog;
+
+typedef struct {
+    int nvar, nfix, nreact, nnz, nb;
+    const double *lit;   /* literal table                                  */
+    or_sumprog rate;     /* A(r)   : one term per reaction                 */
+    or_sumprog prod;     /* P_VAR(i)                                       */
+    or_sumprog dest;     /* D_VAR(i)                                       */
+    or_sumprog jacb;     /* B(k)   : one term (or none for unused k)       */
+    or_sumprog jvs;      /* JVS(k)                                         */
+    const int *crow;     /* [nvar+1] CSR row starts of the LU pattern      */
+    const int *icol;     /* [nnz]                                          */
+    const int *diag;     /* [nvar]                                         */
+    int pow_mode;        /* Err**(1/3): 0 libm pow (the reference CPU      */
+                         /* build), 1 correctly rounded (quad precision)   */
+} or_mech;
+
+/* Work buffers sized by the caller: see or_work_doubles(). */
+long or_work_doubles(const or_mech *m);
+
+void or_fun_split(const or_mech *m, const double *V, const double *F,
+                  const double *RCT, double *Vdot, double *P, double *D,
+                  double *A);
+void or_jac_sp(const or_mech *m, const double *V, const double *F,
+               const double *RCT, double *JVS, double *B);
+void or_decomp(const or_mech *m, double *JVS, double *W, int *ier);
+void or_solve(const or_mech *m, const double *JVS, double *X);
+void or_solve_tr(const or_mech *m, const double *JVS, const double *X,
+                 double *XX);
+
+/* Integrate_Cell_S with the option merge of gpu_solve_cell. */
+void or_solve_cell(const or_mech *m, double *Y, const double *FIX,
+                   const double *RCONST, double Tstart, double Tend,
+                   const double *ATOL, const double *RTOL,
+                   const int *ICNTRL_U, const double *RCNTRL_U,
+                   int *ISTAT, double *RSTAT, int *IERR, double *work);
+
+/* Batched driver with the gc_gpu_integrate_batch argument layout.
+ * nthreads <= 0 uses the OpenMP default. Returns wall seconds of the solve. */
+double or_integrate_batch(const or_mech *m, int nCell, double Tstart,
+                          double Tend, double *C, const double *RCONST,
+                          const double *ATOL, const double *RTOL,
+                          const int *ICNTRL, const double *RCNTRL,
+                          const int *Perm, int *ISTAT, double *RSTAT,
+                          int *IERR, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
