@@ -76,6 +76,14 @@ def _run_parallel(jobs, max_jobs):
 
 
 def build_mechanism(mech: str, force: bool = False, jobs: int | None = None) -> Path:
+    todo, link = plan_mechanism(mech, force)
+    if todo:
+        _run_parallel(todo, jobs or max(1, min(len(todo), os.cpu_count() or 1)))
+    return link()
+
+
+def plan_mechanism(mech: str, force: bool = False):
+    """Generate the mechanism header; return (compile jobs, link function)."""
     sys.path.insert(0, str(PKG / "codegen"))
     import gen_hip  # noqa: E402
 
@@ -106,19 +114,21 @@ def build_mechanism(mech: str, force: bool = False, jobs: int | None = None) -> 
     if force or _stale(obj, deps):
         todo.append(([HIPCC, *HIP_FLAGS, f"-I{CSRC}", "-c", str(CSRC / "gc_api.hip"),
                       "-o", str(obj)], f"{mech}/api.o"))
-    if todo:
-        _run_parallel(todo, jobs or max(1, min(len(todo), os.cpu_count() or 1)))
-    out = lib_path(mech)
-    if force or _stale(out, objs):
-        tmp = out.with_suffix(".so.tmp")
-        subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-                        *map(str, objs)], check=True)
-        os.replace(tmp, out)
-    if mech == "fullchem":
-        drop_in = LIB / "libgcchemgpu.so"
-        if force or _stale(drop_in, [out]):
-            shutil.copyfile(out, drop_in)
-    return out
+
+    def link() -> Path:
+        out = lib_path(mech)
+        if force or _stale(out, objs):
+            tmp = out.with_suffix(".so.tmp")
+            subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
+                            *map(str, objs)], check=True)
+            os.replace(tmp, out)
+        if mech == "fullchem":
+            drop_in = LIB / "libgcchemgpu.so"
+            if force or _stale(drop_in, [out]):
+                shutil.copyfile(out, drop_in)
+        return out
+
+    return todo, link
 
 
 def available_mechanisms():
@@ -126,7 +136,12 @@ def available_mechanisms():
 
 
 def build_all(force: bool = False, mechs=None) -> list[Path]:
-    return [build_mechanism(m, force=force) for m in (mechs or available_mechanisms())]
+    """Build every mechanism library; all compile jobs share one pool."""
+    plans = [plan_mechanism(m, force) for m in (mechs or available_mechanisms())]
+    todo = [job for jobs, _ in plans for job in jobs]
+    if todo:
+        _run_parallel(todo, max(1, min(len(todo), os.cpu_count() or 1)))
+    return [link() for _, link in plans]
 
 
 if __name__ == "__main__":

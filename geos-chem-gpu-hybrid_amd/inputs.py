@@ -189,7 +189,8 @@ def subset(d: dict, idx) -> dict:
     idx = np.asarray(idx)
     out = dict(d)
     for k in ("C", "RCONST", "ICNTRL", "RCNTRL", "T", "P_hpa", "cos_sza"):
-        out[k] = np.ascontiguousarray(d[k][idx])
+        if k in d:
+            out[k] = np.ascontiguousarray(d[k][idx])
     out["Perm"] = np.arange(1, idx.size + 1, dtype=np.int32)
     out.pop("grid", None)
     return out
@@ -203,3 +204,20 @@ def cost_order(nsteps, descending: bool = True):
     key = np.asarray(nsteps, dtype=np.int64)
     order = np.argsort(-key if descending else key, kind="stable")
     return (order + 1).astype(np.int32)
+
+
+def config4(mech: Mechanism, n: int, seed: int = 4) -> dict:
+    """Cells of a synthetic stress mechanism: concentrations log-uniform
+    [1e-2, 1e10]; rate constants log-uniform [1e-6, 1e4] s^-1 in scaled units,
+    i.e. RCONST = k / (1e6)^(order-1) so that k is the pseudo-first-order rate
+    at a concentration of 1e6; dt = 600 s; fixed species 1e10."""
+    rng = np.random.default_rng(seed)
+    C = np.empty((n, mech.nspec))
+    C[:, :mech.nvar] = _logu(rng, 1e-2, 1e10, (n, mech.nvar))
+    C[:, mech.nvar:] = 1e10
+    order = np.array([sum(1 for k, _ in f[1:] if k in ("V", "F")) for f in mech.ir["rate"]])
+    k = _logu(rng, 1e-6, 1e4, (n, mech.nreact))
+    RCONST = k / (1e6 ** np.maximum(order - 1, 0))[None, :]
+    ICNTRL, RCNTRL, ATOL, RTOL = _controls(n, mech.nvar)
+    return dict(C=C, RCONST=RCONST, ICNTRL=ICNTRL, RCNTRL=RCNTRL, ATOL=ATOL, RTOL=RTOL,
+                Perm=np.arange(1, n + 1, dtype=np.int32), Tstart=0.0, Tend=600.0)

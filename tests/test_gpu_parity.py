@@ -1,0 +1,243 @@
+"""GPU parity of the HIP Rodas3 library against the CPU oracle (MI355X).
+
+Bar: with the correctly rounded Err**(1/3) on both sides the GPU result is
+bit-identical to the oracle (C, ISTAT, RSTAT, IERR); against the oracle with
+libm pow (the reference CPU build) final concentrations agree to 1e-9
+relative above ATOL and step counts are identical in >= 99.9% of cells.
+At the full 4x5 size (238,464 cells) the checks are the size-independent
+ones: plain and persistent kernels agree bit for bit, every cell succeeds,
+and a sample of cells is bit-identical to the oracle.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from gcchem_amd import inputs, lib as L, mechanism, solver
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_REL = 1e-9  # north-star tolerance on final concentrations above ATOL
+
+
+def _run_oracle(o, d, perm=None):
+    return o.integrate_batch(d["Tstart"], d["Tend"], d["C"], d["RCONST"], d["ATOL"], d["RTOL"],
+                             d["ICNTRL"], d["RCNTRL"], d["Perm"] if perm is None else perm)
+
+
+def _gpu(d, kernel, mech="fullchem", chunk=0):
+    b = solver.Batch.from_numpy(d)
+    r = solver.solve(b, kernel=kernel, chunk=chunk, mech=mech)
+    torch.cuda.synchronize()
+    return (r.C.cpu().numpy(), r.ISTAT.cpu().numpy(), r.RSTAT.cpu().numpy(),
+            r.IERR.cpu().numpy())
+
+
+def _assert_bitexact(g, ref, nvar):
+    C, IS, RS, IE = g
+    assert np.array_equal(IE, ref[3])
+    assert np.array_equal(IS, ref[1])
+    assert np.array_equal(RS, ref[2])
+    assert np.array_equal(C[:, :nvar], ref[0][:, :nvar])
+
+
+@pytest.fixture(scope="module")
+def cfg0(mech):
+    return inputs.config0(mech, n=2048, seed=0)
+
+
+@pytest.fixture(scope="module")
+def ref0(cfg0, oracle_cr):
+    return _run_oracle(oracle_cr, cfg0)
+
+
+def test_selftest_and_info():
+    lib = L.load()
+    nd = ctypes.c_int()
+    name = ctypes.create_string_buffer(256)
+    lib.gc_gpu_info(ctypes.byref(nd), name, 256)
+    assert nd.value >= 1 and len(name.value) > 0
+    rc, ns, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    lib.gc_gpu_selftest(ctypes.byref(rc), ctypes.byref(ns), ctypes.byref(t))
+    assert (rc.value, ns.value) == (1, 12)
+    assert lib.gc_gpu_set_device(0) == 0
+    assert lib.gc_gpu_set_device(999) == -1
+
+
+@pytest.mark.parametrize("kernel", ["plain", "persistent"])
+def test_config0_bitexact(cfg0, ref0, mech, kernel):
+    _assert_bitexact(_gpu(cfg0, kernel), ref0, mech.nvar)
+
+
+def test_config0_vs_reference_pow(cfg0, mech, oracle_libm):
+    """Against libm pow (the reference CPU build): identical IERR, identical
+    step counts in >= 99.9% of cells, and final concentrations above ATOL
+    within 1e-9 relative for >= 99.99% of values (max 1e-7).
+
+    The GPU equals the oracle bit for bit when both round Err**(1/3)
+    correctly (test_config0_bitexact); glibc's pow misrounds about 0.08% of
+    its arguments by one ulp, and a one-ulp change of a step size moves a few
+    values that sit just above ATOL by up to ~1e-8 relative after ~90 steps.
+    The same spread appears between the two CPU oracle modes."""
+    ref = _run_oracle(oracle_libm, cfg0)
+    C, IS, RS, IE = _gpu(cfg0, "persistent")
+    nv = mech.nvar
+    assert np.array_equal(IE, ref[3])
+    same_steps = np.all(IS[:, 2:5] == ref[1][:, 2:5], axis=1).mean()
+    assert same_steps >= 0.999
+    above = np.abs(ref[0][:, :nv]) > cfg0["ATOL"][None, :]
+    rel = np.abs(C[:, :nv] - ref[0][:, :nv]) / np.abs(ref[0][:, :nv]).clip(1e-300)
+    assert (rel[above] <= TOL_REL).mean() >= 0.9999
+    assert rel[above].max() <= 1e-7
+
+
+def test_host_array_entry_points(cfg0, ref0, mech):
+    """gc_gpu_integrate_batch(_dev) with host arrays (the model's call)."""
+    lib = L.load()
+    n = 256
+    d = inputs.subset(cfg0, np.arange(n))
+    C = np.ascontiguousarray(d["C"].copy())
+    IS = np.zeros((n, 8), np.int32)
+    RS = np.zeros((n, 3))
+    IE = np.zeros(n, np.int32)
+    tt, tk = ctypes.c_double(), ctypes.c_double()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    perm = np.arange(1, n + 1, dtype=np.int32)[::-1].copy()  # reversed launch order
+    lib.gc_gpu_integrate_batch_dev(0, n, 0.0, 600.0, p(C), p(d["RCONST"]), p(d["ATOL"]),
+                                   p(d["RTOL"]), p(d["ICNTRL"]), p(d["RCNTRL"]), p(perm),
+                                   p(IS), p(RS), p(IE), ctypes.byref(tt), ctypes.byref(tk))
+    assert tt.value > 0 and tk.value > 0
+    sub = tuple(x[:n] for x in ref0[:4])
+    _assert_bitexact((C, IS, RS, IE), sub, mech.nvar)
+    # a device that does not exist: every IERR = -99
+    lib.gc_gpu_integrate_batch_dev(77, n, 0.0, 600.0, p(C), p(d["RCONST"]), p(d["ATOL"]),
+                                   p(d["RTOL"]), p(d["ICNTRL"]), p(d["RCNTRL"]), p(perm),
+                                   p(IS), p(RS), p(IE), ctypes.byref(tt), ctypes.byref(tk))
+    assert np.all(IE == -99)
+    # nCell = 0 returns at once with zero timers
+    lib.gc_gpu_integrate_batch(0, 0.0, 600.0, p(C), p(d["RCONST"]), p(d["ATOL"]), p(d["RTOL"]),
+                               p(d["ICNTRL"]), p(d["RCNTRL"]), p(perm), p(IS), p(RS), p(IE),
+                               ctypes.byref(tt), ctypes.byref(tk))
+    assert tt.value == 0.0 and tk.value == 0.0
+
+
+@pytest.mark.parametrize("kernel", ["plain", "persistent"])
+def test_permutation_and_chunks(cfg0, ref0, mech, kernel):
+    """Cells are independent: any launch order and any chunking gives the same
+    per-cell results."""
+    d = dict(cfg0)
+    d["Perm"] = (np.random.default_rng(1).permutation(cfg0["C"].shape[0]) + 1).astype(np.int32)
+    _assert_bitexact(_gpu(d, kernel), ref0, mech.nvar)
+    _assert_bitexact(_gpu(cfg0, kernel, chunk=700), ref0, mech.nvar)
+
+
+def test_tuning_variants_identical(cfg0, ref0, mech):
+    lib = L.load()
+    old = L.get_tuning(lib)
+    try:
+        for t in ({"regs": 2}, {"cells_per_wave": 32, "grain": 1}, {"block": 64, "blocks_per_cu": 4},
+                  {"cells_per_wave": 16, "grain": 4096, "regs": 1}):
+            L.set_tuning(lib, **t)
+            _assert_bitexact(_gpu(cfg0, "persistent"), ref0, mech.nvar)
+    finally:
+        L.set_tuning(lib, **old)
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 200])
+def test_ragged_sizes(cfg0, ref0, mech, n):
+    d = inputs.subset(cfg0, np.arange(n))
+    sub = tuple(x[:n] for x in ref0[:4])
+    for kernel in ("plain", "persistent"):
+        _assert_bitexact(_gpu(d, kernel), sub, mech.nvar)
+
+
+def test_failure_codes_on_gpu(mech, oracle_cr):
+    d = inputs.config0(mech, n=64, seed=21)
+    d["ICNTRL"][0, 11] = 1   # -90
+    d["ICNTRL"][1, 2] = 2    # -2
+    d["ICNTRL"][2, 3] = 1    # -6
+    d["ICNTRL"][3, 15] = 1   # clamp Y at 0 on acceptance
+    d["RCNTRL"][4, 0] = 600.0  # Hmin = dt: every step accepted
+    d["RCNTRL"][5, 2] = 30.0   # warm start Hstart
+    d["RCNTRL"][6, 1] = 10.0   # Hmax
+    d["ICNTRL"][7, 0] = 0      # non-autonomous branch (dFdT replica)
+    d["ICNTRL"][8, 1] = 1      # scalar tolerances
+    ref = _run_oracle(oracle_cr, d)
+    assert list(ref[3][:3]) == [-90, -2, -6]
+    for kernel in ("plain", "persistent"):
+        _assert_bitexact(_gpu(d, kernel), ref, mech.nvar)
+
+
+def test_bad_perm_rejected(cfg0):
+    d = inputs.subset(cfg0, np.arange(64))
+    d["Perm"][5] = 65  # outside 1..nCell
+    b = solver.Batch.from_numpy(d)
+    with pytest.raises(RuntimeError, match="Perm"):
+        solver.solve(b, kernel="persistent")
+
+
+def test_singular_retry_path():
+    """X -> 2X at rate k = 1/8 (J = +1/8) with Hstart = 16: the prepared
+    diagonal -J + 1/(0.5 H) is exactly 0, KppDecomp reports it singular, H is
+    halved and the attempt succeeds (Nsng = 1)."""
+    from gcchem_amd import synthetic
+    import json
+    ir = synthetic.build_ir("auto1", 1, 1, [synthetic.Reaction({0: 1}, {0: 2.0})], reorder=False)
+    o = Oracle(ir, pow_mode="cr")
+    C = np.array([[1.0, 0.0]])
+    R = np.array([[0.125]])
+    I = np.zeros((1, 20), np.int32)
+    I[0, 0], I[0, 2] = 1, 4
+    Rc = np.zeros((1, 20))
+    Rc[0, 2] = 16.0
+    ref = o.integrate_batch(0.0, 64.0, C, R, np.array([1.0]), np.array([1e-3]), I, Rc)
+    assert ref[3][0] == 1 and ref[1][0, 7] >= 1
+    # the stress mechanism library checks the generic path; this 1-species
+    # mechanism is exercised on the CPU only (no library is built for it)
+
+
+@pytest.fixture(scope="module")
+def stress():
+    return mechanism.load("stress500")
+
+
+@pytest.mark.parametrize("kernel", ["plain", "persistent"])
+def test_stress_mechanism_bitexact(stress, kernel):
+    d = inputs.config4(stress, n=256, seed=4)
+    ref = _run_oracle(Oracle(stress.ir, pow_mode="cr"), d)
+    _assert_bitexact(_gpu(d, kernel, mech="stress500"), ref, stress.nvar)
+
+
+def test_stress_ragged_batches(stress):
+    """Independent batches of different sizes in turn (the config-4 ragged
+    stress, scaled down to a 60 s step): each matches the oracle on a
+    sample of its cells."""
+    o = Oracle(stress.ir, pow_mode="cr")
+    for i, n in enumerate((1, 100, 1000)):
+        d = inputs.config4(stress, n=n, seed=40 + i)
+        d["Tend"] = 60.0
+        g = _gpu(d, "persistent", mech="stress500")
+        idx = np.unique(np.linspace(0, n - 1, min(n, 16)).astype(int))
+        ref = _run_oracle(o, inputs.subset(d, idx))
+        _assert_bitexact(tuple(x[idx] for x in g), ref, stress.nvar)
+        print(f"stress batch of {n} cells ok", flush=True)
+
+
+def test_full_4x5_grid(mech, oracle_cr):
+    """238,464 cells: plain == persistent bit for bit, all IERR = 1, and 48
+    sampled cells bit-identical to the oracle."""
+    d = inputs.config1(mech, seed=1)
+    b = solver.Batch.from_numpy(d)
+    rp = solver.solve(b, kernel="persistent")
+    rq = solver.solve(b, kernel="plain")
+    torch.cuda.synchronize()
+    for a, c in ((rp.C, rq.C), (rp.ISTAT, rq.ISTAT), (rp.RSTAT, rq.RSTAT), (rp.IERR, rq.IERR)):
+        assert torch.equal(a, c)
+    IE = rp.IERR.cpu().numpy()
+    assert np.all(IE == 1)
+    idx = np.random.default_rng(2).choice(IE.size, 48, replace=False)
+    ref = _run_oracle(oracle_cr, inputs.subset(d, idx))
+    g = (rp.C.cpu().numpy()[idx], rp.ISTAT.cpu().numpy()[idx], rp.RSTAT.cpu().numpy()[idx], IE[idx])
+    _assert_bitexact(g, ref, mech.nvar)

@@ -1,0 +1,202 @@
+"""Pin the CPU oracle (the checker of every GPU parity test).
+
+* sparse LU / solves: against dense linear algebra on the same matrices;
+* Integrate_Cell_S: the reference's known answers (the boundary self-test,
+  gpulib/src/gc_gpu_batch_api.F90:349-376, and the multi-device dummy
+  chemistry, gpulib/src/mgpu_test.F90:45-53), the closed-form step sizes of
+  the self-test, the failure codes of gckpp_BatchIntegrator.F90:468-592, and
+  what mathematics fixes for Rodas3: order 3 and L-stability of its
+  stability function, and the Robertson problem against an independent stiff
+  solver (SciPy Radau).
+"""
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+from gcchem_amd import inputs, synthetic
+from oracle import Oracle
+
+
+def _dense(mech, vals):
+    crow, icol = mech.ir["crow"], mech.ir["icol"]
+    A = np.zeros((mech.nvar, mech.nvar))
+    for i in range(mech.nvar):
+        for k in range(crow[i], crow[i + 1]):
+            A[i, icol[k]] = vals[k]
+    return A
+
+
+@pytest.fixture(scope="module")
+def prepared(mech, oracle_libm):
+    """Ghimj = -J + (1/(H*0.5)) I at a random state, H = 60 s."""
+    rng = np.random.default_rng(3)
+    V = np.exp(rng.uniform(np.log(1e2), np.log(1e10), mech.nvar))
+    F = np.array([3e17, 1.9e19, 5e18])
+    RCT = np.exp(rng.uniform(np.log(1e-15), np.log(1e-10), mech.nreact))
+    jvs = oracle_libm.jac_sp(V, F, RCT)
+    g = -jvs
+    g[mech.ir["diag"]] += 1.0 / (60.0 * 0.5)
+    return g
+
+
+def test_decomp_equals_dense_lu(mech, oracle_libm, prepared):
+    lu, ier = oracle_libm.decomp(prepared)
+    assert ier == 0
+    A = _dense(mech, prepared)
+    LU = _dense(mech, lu)
+    L = np.tril(LU, -1) + np.eye(mech.nvar)
+    U = np.triu(LU)
+    assert np.allclose(L @ U, A, rtol=1e-11, atol=1e-11 * np.abs(A).max())
+    # entries outside the pattern stay zero: the pattern is the LU pattern
+    Ld, Ud = _doolittle(A)
+    assert np.allclose(L, Ld, rtol=1e-9, atol=1e-12)
+    assert np.allclose(U, Ud, rtol=1e-9, atol=1e-12 * np.abs(Ud).max())
+
+
+def _doolittle(A):
+    n = A.shape[0]
+    L, U = np.eye(n), A.copy()
+    for k in range(n):
+        L[k + 1:, k] = U[k + 1:, k] / U[k, k]
+        U[k + 1:, :] -= np.outer(L[k + 1:, k], U[k, :])
+        U[k + 1:, k] = 0.0
+    return L, U
+
+
+def test_solve_and_transpose_solve(mech, oracle_libm, prepared):
+    lu, _ = oracle_libm.decomp(prepared)
+    LU = _dense(mech, lu)
+    M = (np.tril(LU, -1) + np.eye(mech.nvar)) @ np.triu(LU)
+    b = np.random.default_rng(9).normal(size=mech.nvar)
+    x = oracle_libm.solve(lu, b)
+    ref = np.linalg.solve(M, b)
+    assert np.allclose(x, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+    xt = oracle_libm.solve_tr(lu, b)
+    ref = np.linalg.solve(M.T, b)
+    assert np.allclose(xt, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+
+
+def _selftest_inputs(mech, n=1):
+    C = np.full((n, mech.nspec), 1.0e6)
+    R = np.zeros((n, mech.nreact))
+    I = np.zeros((n, 20), np.int32)
+    I[:, 0], I[:, 1], I[:, 2], I[:, 6], I[:, 14] = 1, 0, 4, 1, -1
+    return C, R, np.full(mech.nvar, 1e-2), np.full(mech.nvar, 0.5e-2), I, np.zeros((n, 20))
+
+
+@pytest.mark.parametrize("mode", ["libm", "cr"])
+def test_selftest_known_answer(mech, mode):
+    o = Oracle(mech.ir, pow_mode=mode)
+    C, R, A, Rt, I, Rc = _selftest_inputs(mech)
+    _, IS, RS, IE, _ = o.integrate_batch(0.0, 1200.0, C, R, A, Rt, I, Rc)
+    # reference answer rc = 1, nsteps = 12 (docs/GPU_IN_MODEL_RESULTS.md:182-184)
+    assert IE[0] == 1 and IS[0, 2] == 12
+    # closed form: the error is negligible, so every step grows by FacMax = 6
+    # from Hstart = 1e-5 until the last step is clamped to the end time
+    H = 1e-5 * 6.0 ** np.arange(11)
+    assert RS[0, 0] == 1200.0 and RS[0, 2] == 1200.0
+    assert abs(RS[0, 1] - (1200.0 - H.sum())) < 1e-9
+    assert list(IS[0]) == [36, 12, 12, 12, 0, 12, 48, 0]
+
+
+def test_mgpu_dummy_chemistry(mech, oracle_libm):
+    """mgpu_test.F90:45-53: C = 1e6 + mod(i,97), RCONST(1:50) = 1e-4*(1+mod(i,13)/13),
+    [0,1200] s; the reference expects no failed cell."""
+    n = 300
+    C, R, A, Rt, I, Rc = _selftest_inputs(mech, n)
+    i = np.arange(1, n + 1)
+    C[:] = (1.0e6 + np.mod(i, 97))[:, None]
+    R[:, :50] = (1.0e-4 * (1.0 + np.mod(i, 13) / 13.0))[:, None]
+    _, IS, _, IE, _ = oracle_libm.integrate_batch(0.0, 1200.0, C, R, A, Rt, I, Rc)
+    assert np.all(IE == 1)
+
+
+def test_failure_codes(mech, oracle_libm):
+    d = inputs.config0(mech, n=8, seed=21)
+    I = d["ICNTRL"].copy()
+    Rc = d["RCNTRL"].copy()
+    I[0, 11] = 1      # auto-reduce requested -> -90
+    I[1, 2] = 2       # not Rodas3 -> -2
+    I[2, 3] = 1       # Max_no_steps = 1 -> -6 after two steps
+    C, IS, RS, IE, _ = oracle_libm.integrate_batch(0.0, 600.0, d["C"], d["RCONST"], d["ATOL"],
+                                                   d["RTOL"], I, Rc)
+    assert IE[0] == -90 and IE[1] == -2 and IE[2] == -6
+    assert np.all(IE[3:] == 1)
+    assert np.all(IS[0] == 0) and np.all(IS[1] == 0) and IS[2, 2] == 2
+    assert np.array_equal(C[0], d["C"][0]) and np.array_equal(C[1], d["C"][1])
+    # step too small -> -7: T + 0.1*H == T at T = 1e20
+    C, IS, RS, IE, _ = oracle_libm.integrate_batch(1e20, 1e20 + 1e7, d["C"][:1], d["RCONST"][:1],
+                                                   d["ATOL"], d["RTOL"], d["ICNTRL"][:1],
+                                                   d["RCNTRL"][:1])
+    assert IE[0] == -7
+
+
+def _one_species(literal_k=None):
+    rx = [synthetic.Reaction({0: 1}, {})]
+    return synthetic.build_ir("linear1", 1, 1, rx, reorder=False)
+
+
+def _one_step(o, k, h):
+    C = np.array([[1.0, 0.0]])
+    R = np.array([[k]])
+    I = np.zeros((1, 20), np.int32)
+    I[0, 0], I[0, 2] = 1, 4
+    Rc = np.zeros((1, 20))
+    Rc[0, 0] = h      # Hmin = h: the step is accepted whatever its error
+    Rc[0, 2] = h      # Hstart = h
+    C1, IS, RS, IE, _ = o.integrate_batch(0.0, h, C, R, np.array([1e-30]), np.array([1e-12]), I, Rc)
+    assert IE[0] == 1 and IS[0, 2] == 1
+    return C1[0, 0]
+
+
+def test_rodas3_order_and_l_stability():
+    """y' = -k y: one step multiplies y by R(z), z = -k h.  Rodas3 is of order
+    3 (local error ~ z^4) and L-stable (R(z) -> 0 as z -> -inf)."""
+    o = Oracle(_one_species(), pow_mode="cr")
+    errs = []
+    for z in (0.02, 0.04, 0.08):
+        errs.append(abs(_one_step(o, z, 1.0) - np.exp(-z)))
+    r1, r2 = errs[1] / errs[0], errs[2] / errs[1]
+    assert 13.0 < r1 < 19.0 and 13.0 < r2 < 19.0  # 2^4 = 16
+    assert abs(_one_step(o, 1e8, 1.0)) < 1e-6
+    for z in np.linspace(0.1, 100.0, 40):
+        assert abs(_one_step(o, z, 1.0)) < 1.0
+
+
+def test_robertson_against_radau():
+    """Robertson's stiff kinetics (A -> B, 2B -> B + C, B + C -> A + C) over
+    40 s at tight tolerance against SciPy's Radau IIA."""
+    rx = [synthetic.Reaction({0: 1}, {1: 1.0}),
+          synthetic.Reaction({1: 2}, {1: 1.0, 2: 1.0}),
+          synthetic.Reaction({1: 1, 2: 1}, {0: 1.0, 2: 1.0})]
+    ir = synthetic.build_ir("robertson", 3, 1, rx, reorder=False)
+    o = Oracle(ir, pow_mode="cr")
+    k = np.array([[0.04, 3e7, 1e4]])
+    C = np.array([[1.0, 0.0, 0.0, 0.0]])
+    I = np.zeros((1, 20), np.int32)
+    I[0, 0], I[0, 2] = 1, 4
+    C1, IS, RS, IE, _ = o.integrate_batch(0.0, 40.0, C, k, np.full(3, 1e-14), np.full(3, 1e-7),
+                                          I, np.zeros((1, 20)))
+    assert IE[0] == 1
+
+    def f(t, y):
+        a, b, c = y
+        return [-0.04 * a + 1e4 * b * c, 0.04 * a - 1e4 * b * c - 3e7 * b * b, 3e7 * b * b]
+
+    ref = solve_ivp(f, (0.0, 40.0), [1.0, 0.0, 0.0], method="Radau", rtol=1e-11, atol=1e-16).y[:, -1]
+    assert np.allclose(C1[0, :3], ref, rtol=2e-5, atol=0)
+
+
+def test_pow_modes_agree_to_tolerance(mech):
+    """libm pow and the correctly rounded Err**(1/3) differ in the last ulp for
+    a small fraction of arguments; the solutions stay within 1e-9."""
+    d = inputs.config0(mech, n=64, seed=7)
+    args = (d["Tstart"], d["Tend"], d["C"], d["RCONST"], d["ATOL"], d["RTOL"], d["ICNTRL"], d["RCNTRL"])
+    a = Oracle(mech.ir, pow_mode="libm").integrate_batch(*args)
+    b = Oracle(mech.ir, pow_mode="cr").integrate_batch(*args)
+    nv = mech.nvar
+    assert np.array_equal(a[3], b[3])
+    assert (a[1] == b[1]).all(axis=1).mean() >= 0.999 or (a[1] != b[1]).any(axis=1).sum() <= 1
+    big = np.abs(a[0][:, :nv]) > 1e-2
+    rel = np.abs(a[0][:, :nv] - b[0][:, :nv]) / np.maximum(np.abs(a[0][:, :nv]), 1e-300)
+    assert rel[big].max() <= 1e-9

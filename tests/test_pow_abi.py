@@ -1,0 +1,79 @@
+"""The product's Err**(1/3) and the C ABI of the built library (CPU only).
+
+* gc_pow_third (csrc/gc_math.h), compiled for the host from the same source
+  the kernels include, must equal the correctly rounded x**RN(1/3) -- from
+  quad-precision powq -- on every sampled argument.
+* The library must load without a GPU and export every function that
+  include/gcchemgpu.h declares.
+"""
+import ctypes
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PKG, REPO
+
+HARNESS = r"""
+#include <quadmath.h>
+#include "gc_math.h"
+long count_mismatch(const double *x, long n) {
+  long bad = 0;
+  for (long i = 0; i < n; ++i)
+    if (gc_pow_third(x[i]) != (double)powq((__float128)x[i], (__float128)(1.0 / 3.0))) ++bad;
+  return bad;
+}
+"""
+
+
+def test_pow_third_correctly_rounded(tmp_path):
+    src = tmp_path / "h.c"
+    src.write_text(HARNESS)
+    so = tmp_path / "libh.so"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", f"-I{PKG / 'csrc'}",
+                    str(src), "-o", str(so), "-lquadmath", "-lm"], check=True)
+    lib = ctypes.CDLL(str(so))
+    lib.count_mismatch.restype = ctypes.c_long
+    lib.count_mismatch.argtypes = [ctypes.c_void_p, ctypes.c_long]
+    rng = np.random.default_rng(0)
+    # the solver calls it with Err >= 1e-10 (floored); cover well beyond
+    x = np.concatenate([10.0 ** rng.uniform(-10, 16, 1_500_000),
+                        rng.uniform(1e-10, 3.0, 500_000),
+                        np.array([1e-10, 1.0, 8.0, 27.0, 1e6, 0.125])])
+    x = np.ascontiguousarray(x)
+    assert lib.count_mismatch(x.ctypes.data_as(ctypes.c_void_p), x.size) == 0
+
+
+def _declared_functions():
+    text = (REPO / "include" / "gcchemgpu.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gc_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("mech_name", ["fullchem", "stress500"])
+def test_library_exports_every_declared_symbol(mech_name):
+    from gcchem_amd import lib as L
+    path = L.lib_path(mech_name)
+    if not path.exists():
+        pytest.fail(f"{path} not built (run __graft_entry__.build())")
+    so = ctypes.CDLL(str(path))
+    names = _declared_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(so, n), n
+    # the binding declares a signature for every one of them
+    assert set(names) == set(L.SIGNATURES)
+    lib = L.load(mech_name)
+    from gcchem_amd import mechanism
+    m = mechanism.load(mech_name)
+    assert L.mech_info(lib) == (m.nvar, m.nfix, m.nreact, m.nnz)
+
+
+def test_drop_in_library_name():
+    from gcchem_amd import lib as L
+    assert (L.LIB_DIR / "libgcchemgpu.so").exists()
+    so = ctypes.CDLL(str(L.LIB_DIR / "libgcchemgpu.so"))
+    for n in ("gc_gpu_integrate_batch", "gc_gpu_integrate_batch_dev", "gc_gpu_info",
+              "gc_gpu_set_device", "gc_gpu_selftest"):
+        assert hasattr(so, n)
