@@ -2,11 +2,12 @@
 
 For each mechanism IR under ``mechanisms/``:
 
-1. ``codegen/gen_hip.py`` writes ``_build/<mech>/mech_gen.inc`` (chunked
-   straight-line FP64 HIP for Fun, Jac_SP+prep, KppDecomp, KppSolve,
-   KppSolveTR);
-2. ``csrc/gc_rodas3.hip`` is compiled once per register-budget variant
-   (``-DGC_WPE=1``: 512 VGPR/lane, ``-DGC_WPE=2``: 256 VGPR/lane) and
+1. ``codegen/gen_tables.py`` writes ``_build/<mech>/mech_tables.inc`` (the
+   operand tables of Fun, Jac_SP + KppDecomp, KppSolve, KppSolveTR and the
+   per-wave workspace layout);
+2. ``csrc/gc_rodas3.hip`` is compiled once per kernel variant
+   (``-DGC_VARIANT=1``: 2 waves per SIMD, 32-entry LU row segments in LDS;
+   ``-DGC_VARIANT=2``: 4 waves per SIMD, 16-entry segments) and
    ``csrc/gc_api.hip`` once, in parallel;
 3. the objects are linked into ``lib/libgcchemgpu_<mech>.so``; the fullchem
    library is also installed as ``lib/libgcchemgpu.so``, the drop-in name the
@@ -39,7 +40,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-VARIANTS = (1, 2)
+VARIANTS = (1, 2)  # kernel variants (GC_VARIANT)
 
 
 def lib_path(mech: str = "fullchem") -> Path:
@@ -85,29 +86,29 @@ def build_mechanism(mech: str, force: bool = False, jobs: int | None = None) -> 
 def plan_mechanism(mech: str, force: bool = False):
     """Generate the mechanism header; return (compile jobs, link function)."""
     sys.path.insert(0, str(PKG / "codegen"))
-    import gen_hip  # noqa: E402
+    import gen_tables  # noqa: E402
 
     ir_path = MECH_DIR / f"{mech}.json"
     bdir = BUILD / mech
     bdir.mkdir(parents=True, exist_ok=True)
     LIB.mkdir(exist_ok=True)
-    inc = bdir / "mech_gen.inc"
-    gen_deps = [ir_path, PKG / "codegen" / "gen_hip.py"]
+    inc = bdir / "mech_tables.inc"
+    gen_deps = [ir_path, PKG / "codegen" / "gen_tables.py"]
     if force or _stale(inc, gen_deps):
         ir = json.loads(ir_path.read_text())
-        text = gen_hip.generate(ir)
+        text = gen_tables.generate(ir)
         if not inc.exists() or inc.read_text() != text:
             inc.write_text(text)
     common = [CSRC / "gc_math.h", CSRC / "gc_rodas3_kernels.h"]
     todo, objs = [], []
-    for w in VARIANTS:
-        obj = bdir / f"rodas3_w{w}.o"
+    for v in VARIANTS:
+        obj = bdir / f"rodas3_v{v}.o"
         objs.append(obj)
         deps = [CSRC / "gc_rodas3.hip", inc] + common
         if force or _stale(obj, deps):
-            todo.append(([HIPCC, *HIP_FLAGS, f"-DGC_WPE={w}", f"-I{CSRC}", f"-I{bdir}",
+            todo.append(([HIPCC, *HIP_FLAGS, f"-DGC_VARIANT={v}", f"-I{CSRC}", f"-I{bdir}",
                           "-c", str(CSRC / "gc_rodas3.hip"), "-o", str(obj)],
-                         f"{mech}/rodas3_w{w}.o"))
+                         f"{mech}/rodas3_v{v}.o"))
     obj = bdir / "api.o"
     objs.append(obj)
     deps = [CSRC / "gc_api.hip", PKG.parent / "include" / "gcchemgpu.h"] + common

@@ -1,0 +1,356 @@
+"""Mechanism IR -> operand tables for the table-driven gfx950 Rodas3 kernels.
+
+The device routines (csrc/gc_rodas3.hip) are small loops that walk these
+tables.  Every table word is wave-uniform, so it arrives through scalar loads
+(one s_load_dwordx16 fetches four records), and every data operand is one
+coalesced ``buffer_load_dwordx2 v, lane_off, rsrc, soffset`` whose soffset is
+the table word itself.  The hot loops run fixed-shape records four at a time,
+so a wave keeps several table fetches and data loads in flight.  The code is
+a few kilobytes whatever the mechanism and stays in the instruction cache.
+
+Per-wave block layout.  The workspace is cut into blocks, one per 64-lane
+wave.  Element ``e`` of lane ``l`` sits at byte ``e*512 + l*8`` of its wave's
+block; the element offsets of the arrays are the ``E_*`` constants.  Tables
+store byte offsets ``e*512``.
+
+Exactness rules (the values must equal the reference's left-to-right
+Fortran evaluation bit for bit):
+
+* products keep their data-factor order; a literal factor is applied first:
+  exact when the literal is the first or second factor (a single multiply
+  commutes) or a power of two anywhere (scaling is exact); anything else is
+  rejected here;
+* a sum term's sign is folded into its coefficient: ``acc - c*x`` equals
+  ``acc + (-c)*x`` and ``-(c*x)`` equals ``(-c)*x`` exactly; terms without a
+  coefficient use 1.0 (``1.0*x == x``);
+* padding entries are exact no-ops: solve pairs ``0.0*0.0`` subtracted from a
+  row value, LU updates into a spare LDS slot.  Sums are never padded (adding
+  +0.0 could flip the sign of a -0.0 accumulator).
+
+Tables (int32 words):
+
+* product groups ``prod_<name>``: 8-word records ``{dst, lit, f0, f1, f2, f3,
+  0, 0}`` sorted by shape (number of data factors 1-4, literal or not);
+  ``prodgrp_<name>`` gives the record count of each of the 8 shapes
+  (shape = (nf-1)*2 + has_lit).  Value = lit*f0*f1*... left to right.
+  Products: ``rate_y``/``rate_yn`` (A(r), V = Y or Ynew; gckpp_Function.F90:
+  2215-3248), ``dprod_y``/``dprod_yn`` (each D_VAR term's product, stored in
+  the DP scratch), ``bjac`` (Jac_SP's B; gckpp_Jacobian.F90:63-3673).
+* ``pd_y``/``pd_yn``: per species 4-word header ``{nP, nD, V(i), i*512}``
+  then nP + nD 4-word terms ``{soff, 0, coef_lo, coef_hi}``: P_VAR(i) over A,
+  D_VAR(i) over DP, Vdot(i) = P_VAR(i) - D_VAR(i)*V(i).
+  (gckpp_Function.F90:3252-5058)
+* ``lu<S>``: fused Jac_SP JVS sums + matrix preparation + KppDecomp with the
+  active row segment (at most S entries, plus one spare slot) in LDS; 4-word
+  ops: ENTRY {1, slot, nterms, diag} + nterms terms; PIVL {2, slot_j, U(j,j),
+  nupd} and PIVS {3, L(k,j), 0, nupd} + nupd/2 4-word update pairs
+  {t, U(j,c), t, U(j,c)} (nupd a multiple of 4); STORE {4, n, 0, 0} + n/2
+  {slot, LU, slot, LU} (n a multiple of 2); END {0,...}.
+  (gckpp_Jacobian.F90:3674-20887, gckpp_BatchIntegrator.F90:573-577,
+  gckpp_LinearAlgebra.F90:46-84)
+* ``fwd<s>``/``bwd<s>`` (stage vector K_s) and ``tr1``/``tr2`` (the TMP
+  vector): solve rows, header {n, x_i, U(i,i) or 0, 0} then n/2
+  {LU, x_j, LU, x_j} with n a multiple of 4, absolute offsets.
+  (gckpp_LinearAlgebra.F90:645-2311, 2327-3701)
+"""
+from __future__ import annotations
+
+import json
+import struct
+import sys
+from pathlib import Path
+
+SEGMENTS = (16, 32)     # LU row-segment sizes compiled (LDS entries per lane)
+
+
+def layout(ir) -> dict:
+    nvar, nfix, nreact, nnz, nb = ir["nvar"], ir["nfix"], ir["nreact"], ir["nnz"], ir["nb"]
+    ndp = max(1, sum(len(s) for s in ir["dest"]))
+    E = {}
+    E["Y"] = 0
+    E["YNEW"] = nvar
+    E["FCN0"] = 2 * nvar
+    E["K"] = 3 * nvar          # 4 stage vectors
+    E["DFDT"] = 7 * nvar
+    E["TMP"] = 8 * nvar        # scratch vector (transposed solves)
+    E["RCT"] = 9 * nvar
+    E["FIX"] = E["RCT"] + nreact
+    E["A"] = E["FIX"] + nfix
+    E["DP"] = E["A"] + nreact  # D_VAR term products
+    E["B"] = E["DP"] + ndp
+    E["LU"] = E["B"] + nb
+    E["ZERO"] = E["LU"] + nnz  # holds 0.0 (padding operand)
+    E["SINK"] = E["ZERO"] + 1  # written by padding stores, never read
+    E["TOTAL"] = E["SINK"] + 1
+    return E
+
+
+def _is_pow2(v: float) -> bool:
+    if v == 0.0 or v != v:
+        return False
+    m, _ = __import__("math").frexp(abs(v))
+    return m == 0.5
+
+
+def _coef_words(c: float):
+    lo, hi = struct.unpack("<II", struct.pack("<d", c))
+    # as signed int32 for the C initialiser
+    return [lo - (1 << 32) if lo >= 1 << 31 else lo, hi - (1 << 32) if hi >= 1 << 31 else hi]
+
+
+class Tables:
+    def __init__(self, ir):
+        self.ir = ir
+        self.E = layout(ir)
+        self.lits: list[float] = []
+        self.lit_index: dict[float, int] = {}
+
+    def off(self, arr: str, k: int) -> int:
+        return (self.E[arr] + k) * 512
+
+    def lit_idx(self, v: float) -> int:
+        if v not in self.lit_index:
+            self.lit_index[v] = len(self.lits)
+            self.lits.append(v)
+        return self.lit_index[v]
+
+    @staticmethod
+    def lit_value(text: str) -> float:
+        return float(text.replace("d", "e").replace("D", "e"))
+
+    def data_off(self, f, vbase: str) -> int:
+        kind, idx = f
+        return self.off({"V": vbase, "F": "FIX", "RCT": "RCT", "A": "A", "B": "B"}[kind], idx)
+
+    # -- products ------------------------------------------------------------
+    def product_shape(self, prod, vbase):
+        """(lit or None, [data offsets]) with the exactness rule of the header."""
+        lits = [n for n, f in enumerate(prod) if f[0] == "lit"]
+        if len(lits) > 1:
+            raise ValueError(f"product with two literals: {prod}")
+        lit = None
+        if lits:
+            n = lits[0]
+            lit = self.lit_value(prod[n][1])
+            if n >= 2 and not _is_pow2(lit):
+                raise ValueError(f"literal in position {n + 1} of {prod} is not exact to move")
+        data = [self.data_off(f, vbase) for f in prod if f[0] != "lit"]
+        if not 1 <= len(data) <= 4:
+            raise ValueError(f"product with {len(data)} data factors: {prod}")
+        return lit, data
+
+    def product_group(self, items):
+        """items: list of (prod, vbase, dst).  Returns (words, counts[8])."""
+        recs = [[] for _ in range(8)]
+        for prod, vbase, dst in items:
+            lit, data = self.product_shape(prod, vbase)
+            shape = (len(data) - 1) * 2 + (1 if lit is not None else 0)
+            li = self.lit_idx(lit) if lit is not None else 0
+            recs[shape].append([dst, li, *data, *([0] * (4 - len(data))), 0, 0])
+        words = [w for group in recs for r in group for w in r]
+        return words, [len(g) for g in recs]
+
+    def rates(self, vbase):
+        return self.product_group([(f, vbase, self.off("A", r))
+                                   for r, f in enumerate(self.ir["rate"])])
+
+    def dprods(self, vbase):
+        items, t = [], 0
+        for i, s in enumerate(self.ir["dest"]):
+            for sign, p in s:
+                items.append((p, vbase, self.off("DP", t)))
+                t += 1
+        return self.product_group(items)
+
+    def bjac(self):
+        return self.product_group([(f, "Y", self.off("B", b))
+                                   for b, f in enumerate(self.ir["jac_b"]) if f is not None])
+
+    # -- sums ------------------------------------------------------------------
+    def sum_term(self, sign, prod, resolve) -> list[int]:
+        """[X] or [lit, X] -> {soff, 0, coef}; coef carries the sign."""
+        if len(prod) == 1:
+            coef, x = 1.0, prod[0]
+        elif len(prod) == 2 and prod[0][0] == "lit":
+            coef, x = self.lit_value(prod[0][1]), prod[1]
+        else:
+            raise ValueError(f"sum term not of the form [lit*]X: {prod}")
+        if sign < 0:
+            coef = -coef
+        return [resolve(x), 0, *_coef_words(coef)]
+
+    def pd(self, vbase) -> list[int]:
+        out, t = [], 0
+        for i in range(self.ir["nvar"]):
+            P, D = self.ir["prod"][i], self.ir["dest"][i]
+            out += [len(P), len(D), self.off(vbase, i), i * 512]
+            for s, p in P:
+                out += self.sum_term(s, p, lambda x: self.data_off(x, vbase))
+            for s, p in D:
+                # the product itself is in DP[t]; the sign stays with the term
+                out += [self.off("DP", t), 0, *_coef_words(-1.0 if s < 0 else 1.0)]
+                t += 1
+        return out
+
+    # -- Jacobian + LU ---------------------------------------------------------
+    def lu(self, seg: int) -> list[int]:
+        ir = self.ir
+        nvar = ir["nvar"]
+        crow, icol, diag, jvs = ir["crow"], ir["icol"], ir["diag"], ir["jvs"]
+        spare = seg * 512
+        ops: list[int] = []
+
+        def pairs(ps, mult, pad):
+            ps = list(ps)
+            while len(ps) % mult:
+                ps.append(pad)
+            w = []
+            for a, b in ps:
+                w += [a, b]
+            return len(ps), w
+
+        for k in range(nvar):
+            cols = icol[crow[k]:crow[k + 1]]
+            pos = {c: crow[k] + n for n, c in enumerate(cols)}
+            for s0 in range(0, len(cols), seg):
+                segcols = cols[s0:s0 + seg]
+                slot = {c: n * 512 for n, c in enumerate(segcols)}
+                lo = segcols[0]
+                for c in segcols:
+                    terms = jvs[pos[c]][2]
+                    ops += [1, slot[c], len(terms), 1 if c == k else 0]
+                    for s, p in terms:
+                        ops += self.sum_term(s, p, lambda x: self.data_off(x, "Y"))
+                for kk in range(crow[k], diag[k]):
+                    j = icol[kk]
+                    if j > segcols[-1]:
+                        break
+                    ups = [(slot[icol[jj]], self.off("LU", jj))
+                           for jj in range(diag[j] + 1, crow[j + 1]) if icol[jj] in slot]
+                    if j in slot:
+                        n, w = pairs(ups, 4, (spare, self.off("ZERO", 0)))
+                        ops += [2, slot[j], self.off("LU", diag[j]), n] + w
+                    elif ups:
+                        assert j < lo
+                        n, w = pairs(ups, 4, (spare, self.off("ZERO", 0)))
+                        ops += [3, self.off("LU", kk), 0, n] + w
+                st = [(slot[c], self.off("LU", pos[c])) for c in segcols]
+                n, w = pairs(st, 2, (spare, self.off("SINK", 0)))
+                ops += [4, n, 0, 0] + w
+        ops += [0, 0, 0, 0]
+        return ops
+
+    # -- solves ----------------------------------------------------------------
+    # One table per target vector, absolute offsets: padding pairs read ZERO
+    # for both operands, so x - 0.0*0.0 = x exactly.
+    def _rows(self, rows) -> list[int]:
+        z = self.off("ZERO", 0)
+        out = []
+        for xi, d, prs in rows:
+            prs = list(prs)
+            while len(prs) % 4:
+                prs.append((z, z))
+            out += [len(prs), xi, d, 0]
+            for a, b in prs:
+                out += [a, b]
+        return out
+
+    def vec(self, vbase: str, stage: int, i: int) -> int:
+        return self.off(vbase, stage * self.ir["nvar"] + i)
+
+    def fwd(self, vbase, stage=0):
+        ir = self.ir
+        crow, icol, diag = ir["crow"], ir["icol"], ir["diag"]
+        x = lambda i: self.vec(vbase, stage, i)
+        rows = [(x(i), 0, [(self.off("LU", j), x(icol[j])) for j in range(crow[i], diag[i])])
+                for i in range(ir["nvar"]) if diag[i] > crow[i]]
+        return self._rows(rows), len(rows)
+
+    def bwd(self, vbase, stage=0):
+        ir = self.ir
+        crow, icol, diag = ir["crow"], ir["icol"], ir["diag"]
+        x = lambda i: self.vec(vbase, stage, i)
+        rows = [(x(i), self.off("LU", diag[i]),
+                 [(self.off("LU", j), x(icol[j])) for j in range(diag[i] + 1, crow[i + 1])])
+                for i in reversed(range(ir["nvar"]))]
+        return self._rows(rows), len(rows)
+
+    def tr(self, vbase="TMP"):
+        """KppSolveTR in place on one vector: phase 1 U^T (rows ascending,
+        U(j,i) for ascending j, divide), phase 2 L^T (rows descending, L(j,i)
+        ascending)."""
+        ir = self.ir
+        n = ir["nvar"]
+        crow, icol, diag = ir["crow"], ir["icol"], ir["diag"]
+        x = lambda i: self.off(vbase, i)
+        ucol = [[] for _ in range(n)]
+        lcol = [[] for _ in range(n)]
+        for j in range(n):
+            for k in range(diag[j] + 1, crow[j + 1]):
+                ucol[icol[k]].append((self.off("LU", k), x(j)))
+            for k in range(crow[j], diag[j]):
+                lcol[icol[k]].append((self.off("LU", k), x(j)))
+        r1 = [(x(i), self.off("LU", diag[i]), ucol[i]) for i in range(n)]
+        r2 = [(x(i), 0, lcol[i]) for i in reversed(range(n)) if lcol[i]]
+        return self._rows(r1), len(r1), self._rows(r2), len(r2)
+
+
+def _arr(name, vals, ctype="int") -> str:
+    body = ",".join(str(v) for v in vals) if vals else "0"
+    return f"__device__ const {ctype} {name}[] = {{{body}}};\n"
+
+
+def generate(ir: dict) -> str:
+    t = Tables(ir)
+    E = t.E
+    parts = [
+        "// GENERATED by codegen/gen_tables.py from the mechanism IR -- do not edit.",
+        f"// mechanism: {ir['name']}  NVAR={ir['nvar']} NFIX={ir['nfix']} "
+        f"NREACT={ir['nreact']} LU_NONZERO={ir['nnz']}",
+        "#pragma once",
+        f'#define MECH_NAME "{ir["name"]}"',
+        f"#define MECH_NVAR {ir['nvar']}",
+        f"#define MECH_NFIX {ir['nfix']}",
+        f"#define MECH_NREACT {ir['nreact']}",
+        f"#define MECH_NNZ {ir['nnz']}",
+    ]
+    for k, v in E.items():
+        parts.append(f"#define MECH_E_{k} {v}")
+    body = []
+    sizes = {}
+    for name, (words, counts) in {"rate_y": t.rates("Y"), "rate_yn": t.rates("YNEW"),
+                                  "dprod_y": t.dprods("Y"), "dprod_yn": t.dprods("YNEW"),
+                                  "bjac": t.bjac()}.items():
+        body.append(_arr(f"gc_prod_{name}", words))
+        body.append(_arr(f"gc_prodgrp_{name}", counts))
+        sizes[name] = len(words)
+    pd_y, pd_yn = t.pd("Y"), t.pd("YNEW")
+    body += [_arr("gc_pd_y", pd_y), _arr("gc_pd_yn", pd_yn)]
+    nf = nbw = 0
+    for stage in range(4):
+        fw, nf = t.fwd("K", stage)
+        bw, nbw = t.bwd("K", stage)
+        body += [_arr(f"gc_fwd{stage}", fw), _arr(f"gc_bwd{stage}", bw)]
+        sizes[f"fwd{stage}"] = len(fw)
+    tr1, n1, tr2, n2 = t.tr("TMP")
+    body += [_arr("gc_tr1", tr1), _arr("gc_tr2", tr2)]
+    for s in SEGMENTS:
+        ops = t.lu(s)
+        body.append(_arr(f"gc_lu{s}", ops))
+        sizes[f"lu{s}"] = len(ops)
+    body.insert(0, _arr("gc_lits", [repr(x) for x in t.lits] or ["0.0"], "double"))
+    parts += [f"#define MECH_NFWD {nf}", f"#define MECH_NBWD {nbw}",
+              f"#define MECH_NTR1 {n1}", f"#define MECH_NTR2 {n2}"]
+    sizes.update(pd=len(pd_y))
+    parts.append("// table words: " + ", ".join(f"{k}={v}" for k, v in sizes.items()))
+    return "\n".join(parts) + "\n" + "".join(body)
+
+
+def main(argv):
+    ir = json.loads(Path(argv[1]).read_text())
+    Path(argv[2]).write_text(generate(ir))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))

@@ -30,27 +30,33 @@ void set_error(const char* where, hipError_t e) {
 }
 void set_error(const char* msg) { g_last_error = msg; }
 
-GcMechDims mech_dims() {
-  static GcMechDims d = [] {
-    GcMechDims x;
-    gc_mech_dims_w1(&x);
-    return x;
-  }();
-  return d;
+GcVariantInfo variant_info(int v) {
+  GcVariantInfo x;
+  if (v == 2) gc_variant_info_v2(&x);
+  else gc_variant_info_v1(&x);
+  return x;
 }
 
-long ws_per_slot() {
-  const GcMechDims d = mech_dims();
-  return GC_WS_DOUBLES_PER_SLOT(d.nvar, d.nfix, d.nreact, d.nnz);
+struct MechDims {
+  int nvar, nfix, nreact, nnz;
+};
+
+MechDims mech_dims() {
+  const GcVariantInfo v = variant_info(1);
+  return MechDims{v.nvar, v.nfix, v.nreact, v.nnz};
 }
+
+long ws_per_slot() { return variant_info(1).ws_doubles_per_slot; }
 
 // ---- launch tuning (process-wide) ------------------------------------------
+// blocks_per_cu = 0 means "fill the CU": as many resident work-groups as the
+// variant's occupancy target allows (waves_per_eu * 4 SIMDs per CU).
 struct Tuning {
   int block = 256;
-  int blocks_per_cu = 1;
+  int blocks_per_cu = 0;
   int cells_per_wave = 64;
   int grain = 64;
-  int regs = 1;
+  int variant = 2;
 };
 std::mutex g_tune_mu;
 Tuning g_tune;
@@ -60,15 +66,15 @@ Tuning current_tuning() {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   if (!g_tune_env_read) {
     g_tune_env_read = true;
-    // GC_AMD_TUNE="block,blocks_per_cu,cells_per_wave,grain,regs"
+    // GC_AMD_TUNE="block,blocks_per_cu,cells_per_wave,grain,variant"
     if (const char* s = getenv("GC_AMD_TUNE")) {
       int v[5] = {0, 0, 0, 0, 0};
       if (sscanf(s, "%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4]) == 5) {
         if (v[0] >= 64 && v[0] <= 256 && v[0] % 64 == 0) g_tune.block = v[0];
-        if (v[1] >= 1 && v[1] <= 32) g_tune.blocks_per_cu = v[1];
+        if (v[1] >= 0 && v[1] <= 32) g_tune.blocks_per_cu = v[1];
         if (v[2] >= 1 && v[2] <= 64) g_tune.cells_per_wave = v[2];
         if (v[3] >= 1 && v[3] <= 4096) g_tune.grain = v[3];
-        if (v[4] == 1 || v[4] == 2) g_tune.regs = v[4];
+        if (v[4] == 1 || v[4] == 2) g_tune.variant = v[4];
       }
     }
   }
@@ -108,6 +114,7 @@ struct DevCtx {
   // staging for the host-array entry points
   DevBuf C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_in = nullptr;  // orders the library stream after the caller's
 };
 
 constexpr int kMaxDev = 64;
@@ -119,6 +126,7 @@ hipError_t ctx_init(DevCtx& c, int dev) {
   if (e != hipSuccess) return e;
   if ((e = hipEventCreate(&c.ev0)) != hipSuccess) return e;
   if ((e = hipEventCreate(&c.ev1)) != hipSuccess) return e;
+  if ((e = hipEventCreateWithFlags(&c.ev_in, hipEventDisableTiming)) != hipSuccess) return e;
   hipDeviceProp_t prop;
   if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) return e;
   c.n_cu = prop.multiProcessorCount;
@@ -154,22 +162,36 @@ hipError_t launch_chunk(DevCtx& c, int kernel, const Tuning& t, const GcBatchArg
                         unsigned S, hipStream_t st) {
   double* ws = (double*)c.ws.p;
   if (kernel == GC_AMD_KERNEL_PLAIN) {
-    return gc_launch_plain_w1(&b, ws, S, 128, st);
+    // the reference's launch: one thread per cell, 128-thread work-groups
+    return gc_launch_plain_v1(&b, ws, 128, st);
   }
   unsigned* q = (unsigned*)c.queue.p;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, q, 1);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (t.regs == 2)
-    return gc_launch_persistent_w2(&b, ws, S, q, t.block, t.cells_per_wave, t.grain, st);
-  return gc_launch_persistent_w1(&b, ws, S, q, t.block, t.cells_per_wave, t.grain, st);
+  // A wave claims `grain` queue positions per atomic.  Never let one claim
+  // exceed the wave's fair share of the batch, or a few waves would take every
+  // cell while the rest of the grid idles.
+  const unsigned waves = S / 64u;
+  unsigned fair = ((unsigned)b.nCell + waves - 1u) / waves;
+  if (fair < 1u) fair = 1u;
+  const int grain = (unsigned)t.grain > fair ? (int)fair : t.grain;
+  if (t.variant == 2)
+    return gc_launch_persistent_v2(&b, ws, S, q, t.block, t.cells_per_wave, grain, st);
+  return gc_launch_persistent_v1(&b, ws, S, q, t.block, t.cells_per_wave, grain, st);
 }
 
 // Number of slots (lanes with workspace) for a launch of n cells.
 unsigned slots_for(DevCtx& c, int kernel, const Tuning& t, int n) {
   if (kernel == GC_AMD_KERNEL_PLAIN) return ((unsigned)n + 63u) & ~63u;
   // persistent: one resident grid; no more waves than there are cells to hand out
-  const unsigned grid_cap = (unsigned)(c.n_cu * t.blocks_per_cu);
+  int bpc = t.blocks_per_cu;
+  if (bpc <= 0) {
+    const int waves_per_cu = 4 * variant_info(t.variant).waves_per_eu;
+    bpc = waves_per_cu / (t.block / 64);
+    if (bpc < 1) bpc = 1;
+  }
+  const unsigned grid_cap = (unsigned)(c.n_cu * bpc);
   const unsigned cells_per_block = (unsigned)(t.block / 64 * t.cells_per_wave);
   unsigned need = ((unsigned)n + cells_per_block - 1u) / cells_per_block;
   if (need < 1u) need = 1u;
@@ -342,7 +364,7 @@ void gc_gpu_integrate_batch_dev(int dev, int nCell, double Tstart, double Tend, 
   DevCtx& c = *cp;
   std::lock_guard<std::mutex> lk(c.mu);
   hipError_t e = ctx_init(c, d);
-  const GcMechDims md = mech_dims();
+  const MechDims md = mech_dims();
   const size_t n = (size_t)nCell;
   const size_t nspec = (size_t)(md.nvar + md.nfix);
   struct { DevBuf* b; const void* h; size_t bytes; } in[] = {
@@ -405,7 +427,7 @@ void gc_gpu_integrate_batch(int nCell, double Tstart, double Tend, double* C,
 }
 
 void gc_gpu_selftest(int* rc, int* nsteps, double* tTotal) {
-  const GcMechDims md = mech_dims();
+  const MechDims md = mech_dims();
   std::vector<double> C(md.nvar + md.nfix, 1.0e6), RCONST(md.nreact, 0.0);
   std::vector<double> ATOL(md.nvar, 1.0e-2), RTOL(md.nvar, 0.5e-2), RCNTRL(20, 0.0), RSTAT(3);
   std::vector<int> ICNTRL(20, 0), ISTAT(8), IERR(1), Perm(1, 1);
@@ -440,37 +462,45 @@ int gc_amd_integrate_batch_device(int kernel, int chunk, int nCell, double Tstar
     return -3;
   }
   GcBatchArgs b{nCell, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR};
-  hipStream_t st = stream ? (hipStream_t)stream : cp->stream;
-  return device_solve(*cp, kernel, chunk, b, st, tKernel);
+  // The solve runs on the library's stream, ordered after all work already
+  // queued on the caller's stream (NULL: the null stream), so inputs the
+  // caller has just written on its stream are complete before any kernel
+  // reads them.  The call is blocking, so later caller work sees the outputs.
+  if ((e = hipEventRecord(cp->ev_in, (hipStream_t)stream)) != hipSuccess ||
+      (e = hipStreamWaitEvent(cp->stream, cp->ev_in, 0)) != hipSuccess) {
+    set_error("ordering after the caller's stream", e);
+    return -3;
+  }
+  return device_solve(*cp, kernel, chunk, b, cp->stream, tKernel);
 }
 
-int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave, int grain, int regs) {
+int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave, int grain, int variant) {
   (void)current_tuning();  // apply GC_AMD_TUNE first so explicit values win
   if ((block > 0 && (block < 64 || block > 256 || block % 64 != 0)) ||
       (blocks_per_cu > 32) || (cells_per_wave > 64) || (grain > 4096) ||
-      (regs > 0 && regs != 1 && regs != 2))
+      (variant > 0 && variant != 1 && variant != 2))
     return -1;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   if (block > 0) g_tune.block = block;
-  if (blocks_per_cu > 0) g_tune.blocks_per_cu = blocks_per_cu;
+  if (blocks_per_cu >= 0) g_tune.blocks_per_cu = blocks_per_cu;
   if (cells_per_wave > 0) g_tune.cells_per_wave = cells_per_wave;
   if (grain > 0) g_tune.grain = grain;
-  if (regs > 0) g_tune.regs = regs;
+  if (variant > 0) g_tune.variant = variant;
   return 0;
 }
 
 void gc_amd_get_tuning(int* block, int* blocks_per_cu, int* cells_per_wave, int* grain,
-                       int* regs) {
+                       int* variant) {
   const Tuning t = current_tuning();
   *block = t.block;
   *blocks_per_cu = t.blocks_per_cu;
   *cells_per_wave = t.cells_per_wave;
   *grain = t.grain;
-  *regs = t.regs;
+  *variant = t.variant;
 }
 
 void gc_amd_mech_info(int* nvar, int* nfix, int* nreact, int* lu_nonzero) {
-  const GcMechDims d = mech_dims();
+  const MechDims d = mech_dims();
   *nvar = d.nvar;
   *nfix = d.nfix;
   *nreact = d.nreact;

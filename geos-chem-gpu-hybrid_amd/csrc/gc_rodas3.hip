@@ -1,46 +1,61 @@
 // gc_rodas3.hip -- batched KPP Rosenbrock Rodas3 on gfx950, one cell per lane.
 //
-// Two launch forms share the generated mechanism kernels (Fun, Jac_SP + matrix
-// preparation, KppDecomp, KppSolve; see codegen/gen_hip.py):
+// Mechanism kernels (Fun_SPLIT, Jac_SP fused with the matrix preparation and
+// KppDecomp, KppSolve) are small loops that walk operand tables generated from
+// the mechanism (codegen/gen_tables.py).  Every table word is wave-uniform and
+// arrives through scalar loads; every data operand is one coalesced
+// buffer_load_dwordx2 with the table word as soffset.  The code is a few KB
+// whatever the mechanism, so it stays resident in the instruction cache.
 //
+// Two launch forms share these routines:
 //  * plain      one thread per cell in Perm order, the nested time / reject /
-//               singular loops of Integrate_Cell_S per thread, exactly the
+//               singular loops of Integrate_Cell_S per thread -- the
 //               reference's batched launch (gpulib/src/gc_gpu_batch_api.F90:
 //               198-215, one `gang vector` thread per cell).
-//  * persistent a fixed grid of lanes that pull cells from a device work queue
-//               in Perm (cost) order.  The per-cell loops are flattened into a
-//               per-lane state machine whose unit of work is one matrix
-//               factorisation attempt, so lanes of a wave at different steps,
-//               retries or cells still share every attempt; a lane whose cell
-//               finishes refills from the queue at once.
+//  * persistent a fixed grid of lanes pulling cells from a device work queue
+//               in Perm (cost) order; the per-cell loops are flattened into a
+//               per-lane state machine whose unit of work is one factorisation
+//               attempt, so lanes at different steps, retries or cells share
+//               every attempt, and a lane whose cell finishes refills at once.
 //
 // Numerics are those of Integrate_Cell_S
 // (src/geos-chem/KPP/fullchem/gckpp_BatchIntegrator.F90:408-707) with the option
 // merge of gpu_solve_cell (gpulib/src/gc_gpu_batch_api.F90:54-110).  Two
 // bit-neutral changes: the Jacobian is re-evaluated at every factorisation
 // attempt instead of being kept as Jac0 (Y is unchanged between a step's
-// attempts, so the values are identical and 45 KB of state per cell is saved),
-// and KppDecomp's singularity test is applied to the prepared diagonal when it
-// is stored (the reference tests each row's diagonal before that row is
-// eliminated, i.e. the same stored value).  Err**(1/3) uses the correctly
-// rounded gc_pow_third (gc_math.h).
+// attempts, so the values are identical and 45 KB of state per cell is
+// saved), and KppDecomp's singularity test is applied to each prepared
+// diagonal entry when it is formed (the reference tests each row's diagonal
+// before that row is eliminated, i.e. the same value; a singular matrix is
+// discarded in both).  Err**(1/3) uses the correctly rounded gc_pow_third.
 //
-// Cell state lives in a slot-major SoA workspace: array a of slot s, element k,
-// at ws[(off_a + k) * S + s].  Caller arrays keep the reference's Fortran
-// layout (cell-major: C(NSPEC,nCell), RCONST(NREACT,nCell), ...).
+// Workspace: one block per 64-lane wave; element e of lane l at byte
+// e*512 + l*8 of the block (MECH_E_* element offsets).  Caller arrays keep the
+// reference's Fortran layout (cell-major C(NSPEC,nCell), RCONST(NREACT,nCell)).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "gc_math.h"
-#include "mech_gen.inc"
+#include "mech_tables.inc"
 #include "gc_rodas3_kernels.h"
 
-#ifndef GC_WPE
-#define GC_WPE 1
+#ifndef GC_VARIANT
+#define GC_VARIANT 1
+#endif
+#if GC_VARIANT == 1
+#define GC_WPE 2
+#define GC_SEG 32
+#define GC_LU_TABLE gc_lu32
+#elif GC_VARIANT == 2
+#define GC_WPE 4
+#define GC_SEG 16
+#define GC_LU_TABLE gc_lu16
+#else
+#error "unknown GC_VARIANT"
 #endif
 #define GC_CAT2(a, b) a##b
 #define GC_CAT(a, b) GC_CAT2(a, b)
-#define GC_SFX(name) GC_CAT(name, GC_CAT(_w, GC_WPE))
+#define GC_SFX(name) GC_CAT(name, GC_CAT(_v, GC_VARIANT))
 
 namespace {
 
@@ -48,7 +63,17 @@ constexpr int NVAR = MECH_NVAR;
 constexpr int NFIX = MECH_NFIX;
 constexpr int NSPEC = MECH_NVAR + MECH_NFIX;
 constexpr int NREACT = MECH_NREACT;
-constexpr int NNZ = MECH_NNZ;
+constexpr int E_TOTAL = MECH_E_TOTAL;
+
+// element byte offsets within a wave block
+constexpr int O_Y = MECH_E_Y * 512;
+constexpr int O_YNEW = MECH_E_YNEW * 512;
+constexpr int O_FCN0 = MECH_E_FCN0 * 512;
+constexpr int O_K = MECH_E_K * 512;
+constexpr int O_DFDT = MECH_E_DFDT * 512;
+constexpr int O_RCT = MECH_E_RCT * 512;
+constexpr int O_FIX = MECH_E_FIX * 512;
+constexpr int O_VEC = NVAR * 512;  // byte stride between consecutive NVAR vectors
 
 // gckpp_BatchIntegrator.F90:66-69
 constexpr double ROUNDOFF = 2.220446049250313080847263336181640625e-16;
@@ -62,30 +87,212 @@ __constant__ double ros_M[4] = {2.0, 0.0, 1.0, 1.0};
 __constant__ double ros_E[4] = {0.0, 0.0, 0.0, 1.0};
 __constant__ double ros_Gamma[4] = {0.5, 1.5, 0.0, 0.0};
 
-// workspace layout (element offsets; multiply by S)
-constexpr long W_Y = 0;
-constexpr long W_YNEW = W_Y + NVAR;
-constexpr long W_FCN0 = W_YNEW + NVAR;
-constexpr long W_K = W_FCN0 + NVAR;
-constexpr long W_DFDT = W_K + 4L * NVAR;
-constexpr long W_RCT = W_DFDT + NVAR;
-constexpr long W_FIX = W_RCT + NREACT;
-constexpr long W_A = W_FIX + NFIX;
-constexpr long W_LU = W_A + NREACT;
-constexpr long W_TOTAL = W_LU + NNZ;
-static_assert(W_TOTAL == GC_WS_DOUBLES_PER_SLOT(NVAR, NFIX, NREACT, NNZ), "layout");
+typedef __attribute__((address_space(4))) const int ctab_t;
+typedef __attribute__((address_space(4))) const double clit_t;
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }
 
-struct Ws {
-  double* base;
-  unsigned S;
-  unsigned slot;
-  __device__ double* arr(long off) const { return base + off * (long)S; }
-  __device__ double ld(long off, int k) const { return base[(off + k) * (long)S + slot]; }
-  __device__ void st(long off, int k, double v) const { base[(off + k) * (long)S + slot] = v; }
+__device__ __forceinline__ ctab_t* tab(const int* p) { return (ctab_t*)p; }
+
+// The calling wave's workspace block and this lane's place in it.
+struct Blk {
+  __amdgpu_buffer_rsrc_t r;
+  int voff;     // lane*8
+  double* lds;  // this lane's slot-0 address in the wave's LDS row segment
+  __device__ __forceinline__ double ld(int soff) const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  }
+  __device__ __forceinline__ void st(int soff, double v) const {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+  }
+  __device__ __forceinline__ double lds_ld(int off) const {
+    return *(double*)((char*)lds + off);
+  }
+  __device__ __forceinline__ void lds_st(int off, double v) const { *(double*)((char*)lds + off) = v; }
 };
+
+__device__ __forceinline__ Blk make_blk(double* ws, unsigned wave, int lane, double* lds_wave) {
+  const unsigned long long base =
+      (unsigned long long)(ws + (size_t)wave * (size_t)E_TOTAL * 64ull);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)base);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
+  void* p = (void*)(((unsigned long long)hi << 32) | lo);
+  Blk b;
+  b.r = __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)(E_TOTAL * 512), 0x00020000);
+  b.voff = lane * 8;
+  b.lds = lds_wave + lane;
+  return b;
+}
+
+__device__ __forceinline__ double coef_of(ctab_t* t) {
+  const unsigned long long lo = (unsigned)t[2], hi = (unsigned)t[3];
+  return __builtin_bit_cast(double, lo | (hi << 32));
+}
+
+// One product record {dst, lit, f0, f1, f2, f3, 0, 0} of shape (NF, LIT):
+// lit*f0*f1*... left to right.
+template <int NF, bool LIT>
+__device__ __forceinline__ double prod_rec(const Blk& b, ctab_t* rec) {
+  double v = b.ld(rec[2]);
+  if (LIT) v = ((clit_t*)gc_lits)[rec[1]] * v;
+  if (NF > 1) v = v * b.ld(rec[3]);
+  if (NF > 2) v = v * b.ld(rec[4]);
+  if (NF > 3) v = v * b.ld(rec[5]);
+  return v;
+}
+
+template <int NF, bool LIT>
+__device__ __forceinline__ ctab_t* prod_group(const Blk& b, ctab_t* rec, int n) {
+  int r = 0;
+  for (; r + 2 <= n; r += 2, rec += 16) {
+    const double v0 = prod_rec<NF, LIT>(b, rec);
+    const double v1 = prod_rec<NF, LIT>(b, rec + 8);
+    b.st(rec[0], v0);
+    b.st(rec[8], v1);
+  }
+  if (r < n) {
+    b.st(rec[0], prod_rec<NF, LIT>(b, rec));
+    rec += 8;
+  }
+  return rec;
+}
+
+// Product records grouped by shape (counts in grp[0..7], shape =
+// (nf-1)*2 + has_lit): rates A(r), D_VAR term products, B temporaries.
+__device__ void run_products(const Blk& b, const int* table, const int* grp_) {
+  ctab_t* rec = tab(table);
+  ctab_t* grp = tab(grp_);
+  rec = prod_group<1, false>(b, rec, grp[0]);
+  rec = prod_group<1, true>(b, rec, grp[1]);
+  rec = prod_group<2, false>(b, rec, grp[2]);
+  rec = prod_group<2, true>(b, rec, grp[3]);
+  rec = prod_group<3, false>(b, rec, grp[4]);
+  rec = prod_group<3, true>(b, rec, grp[5]);
+  rec = prod_group<4, false>(b, rec, grp[6]);
+  prod_group<4, true>(b, rec, grp[7]);
+}
+
+// Sum of n >= 1 terms {soff, 0, coef}: acc = c0*x0, then acc = acc + c*x in
+// order (signs are folded into the coefficients).  Advances t past the terms.
+__device__ __forceinline__ double sum_terms(const Blk& b, ctab_t*& t, int n) {
+  double acc = coef_of(t) * b.ld(t[0]);
+  t += 4;
+  int k = 1;
+  for (; k + 4 <= n; k += 4, t += 16) {
+    const double x0 = b.ld(t[0]), x1 = b.ld(t[4]), x2 = b.ld(t[8]), x3 = b.ld(t[12]);
+    acc = acc + coef_of(t) * x0;
+    acc = acc + coef_of(t + 4) * x1;
+    acc = acc + coef_of(t + 8) * x2;
+    acc = acc + coef_of(t + 12) * x3;
+  }
+  for (; k < n; ++k, t += 4) acc = acc + coef_of(t) * b.ld(t[0]);
+  return acc;
+}
+
+// Fun_SPLIT (gckpp_Function.F90:2170-5060): rates, D_VAR term products, then
+// per species P_VAR, D_VAR and Vdot = P_VAR - D_VAR*V, stored at out + i*512.
+__device__ void fun(const Blk& b, bool ynew, int out) {
+  if (ynew) {
+    run_products(b, gc_prod_rate_yn, gc_prodgrp_rate_yn);
+    run_products(b, gc_prod_dprod_yn, gc_prodgrp_dprod_yn);
+  } else {
+    run_products(b, gc_prod_rate_y, gc_prodgrp_rate_y);
+    run_products(b, gc_prod_dprod_y, gc_prodgrp_dprod_y);
+  }
+  ctab_t* t = tab(ynew ? gc_pd_yn : gc_pd_y);
+  for (int i = 0; i < NVAR; ++i) {
+    const int nP = t[0], nD = t[1], vo = t[2], oo = t[3];
+    t += 4;
+    const double P = nP > 0 ? sum_terms(b, t, nP) : 0.0;
+    const double D = nD > 0 ? sum_terms(b, t, nD) : 0.0;
+    b.st(out + oo, P - D * b.ld(vo));
+  }
+}
+
+// Jac_SP fused with Ghimj = -Jac0, diag += ghinv and KppDecomp, row segments
+// in LDS (the lu table format is described in codegen/gen_tables.py).
+// Returns 1 when a prepared diagonal entry is below TINY.
+__device__ int jac_lu(const Blk& b, double ghinv) {
+  run_products(b, gc_prod_bjac, gc_prodgrp_bjac);
+  ctab_t* op = tab(GC_LU_TABLE);
+  int sing = 0;
+  for (;;) {
+    const int code = op[0];
+    if (code == 0) break;
+    if (code == 1) {
+      const int slot = op[1], nt = op[2], isdiag = op[3];
+      op += 4;
+      const double v = nt > 0 ? sum_terms(b, op, nt) : 0.0;
+      double g = -v;
+      if (isdiag) {
+        g = g + ghinv;
+        sing |= (__builtin_fabs(g) < TINYD) ? 1 : 0;
+      }
+      b.lds_st(slot, g);
+    } else if (code == 2 || code == 3) {
+      double a;
+      if (code == 2) {
+        const int sj = op[1];
+        a = -b.lds_ld(sj) / b.ld(op[2]);
+        b.lds_st(sj, -a);
+      } else {
+        a = -b.ld(op[1]);
+      }
+      const int n = op[3];
+      op += 4;
+      // n is a multiple of 4; padding entries update the spare slot
+      for (int u = 0; u < n; u += 4, op += 8) {
+        const double u0 = b.ld(op[1]), u1 = b.ld(op[3]), u2 = b.ld(op[5]), u3 = b.ld(op[7]);
+        const double w0 = b.lds_ld(op[0]), w1 = b.lds_ld(op[2]);
+        const double w2 = b.lds_ld(op[4]), w3 = b.lds_ld(op[6]);
+        b.lds_st(op[0], w0 + a * u0);
+        b.lds_st(op[2], w1 + a * u1);
+        b.lds_st(op[4], w2 + a * u2);
+        b.lds_st(op[6], w3 + a * u3);
+      }
+    } else {  // code 4: store the segment (pairs; padding writes the sink)
+      const int n = op[1];
+      op += 4;
+      for (int u = 0; u < n; u += 2, op += 4) {
+        const double v0 = b.lds_ld(op[0]), v1 = b.lds_ld(op[2]);
+        b.st(op[1], v0);
+        b.st(op[3], v1);
+      }
+    }
+  }
+  return sing;
+}
+
+// Rows {n, x_i, diag, 0} + n/2 {LU, x_j, LU, x_j} (n a multiple of 4):
+// x_i = x_i - LU*x_j ... in order, then / diag for the backward rows.
+template <bool DIV>
+__device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows) {
+  for (int r = 0; r < rows; ++r) {
+    const int n = t[0], xi = t[1], dg = t[2];
+    t += 4;
+    double x = b.ld(xi);
+    for (int u = 0; u < n; u += 4, t += 8) {
+      const double l0 = b.ld(t[0]), x0 = b.ld(t[1]), l1 = b.ld(t[2]), x1 = b.ld(t[3]);
+      const double l2 = b.ld(t[4]), x2 = b.ld(t[5]), l3 = b.ld(t[6]), x3 = b.ld(t[7]);
+      x = x - l0 * x0;
+      x = x - l1 * x1;
+      x = x - l2 * x2;
+      x = x - l3 * x3;
+    }
+    if (DIV) x = x / b.ld(dg);
+    b.st(xi, x);
+  }
+}
+
+// KppSolve (gckpp_LinearAlgebra.F90:645-2311) in place on stage vector K_s.
+__device__ void solve(const Blk& b, int s) {
+  const int* fw = s == 0 ? gc_fwd0 : s == 1 ? gc_fwd1 : s == 2 ? gc_fwd2 : gc_fwd3;
+  const int* bw = s == 0 ? gc_bwd0 : s == 1 ? gc_bwd1 : s == 2 ? gc_bwd2 : gc_bwd3;
+  solve_rows<false>(b, tab(fw), MECH_NFWD);
+  solve_rows<true>(b, tab(bw), MECH_NBWD);
+}
 
 // Options after the merge of gc_gpu_batch_api.F90:79-88 and the decode of
 // gckpp_BatchIntegrator.F90:468-518.
@@ -122,81 +329,77 @@ __device__ int decode_opts(const int* icu, const double* rcu, double Tstart, dou
   return 0;
 }
 
-// Copy cell `cell` (0-based column of the caller arrays) into the slot.
-__device__ void gather(const GcBatchArgs& b, long cell, const Ws& w) {
-  const double* c = b.C + cell * NSPEC;
-  const double* r = b.RCONST + cell * NREACT;
-  for (int i = 0; i < NVAR; ++i) w.st(W_Y, i, c[i]);
-  for (int j = 0; j < NFIX; ++j) w.st(W_FIX, j, c[NVAR + j]);
-  for (int k = 0; k < NREACT; ++k) w.st(W_RCT, k, r[k]);
+// Copy cell `cell` (0-based column of the caller arrays) into the lane.
+__device__ void gather(const GcBatchArgs& a, long cell, const Blk& b) {
+  const double* c = a.C + cell * NSPEC;
+  const double* r = a.RCONST + cell * NREACT;
+  for (int i = 0; i < NVAR; ++i) b.st(O_Y + i * 512, c[i]);
+  for (int j = 0; j < NFIX; ++j) b.st(O_FIX + j * 512, c[NVAR + j]);
+  for (int k = 0; k < NREACT; ++k) b.st(O_RCT + k * 512, r[k]);
+  b.st(MECH_E_ZERO * 512, 0.0);  // padding operand of the solve tables
 }
 
-__device__ void scatter(const GcBatchArgs& b, long cell, const Ws& w, const int* ist,
+__device__ void scatter(const GcBatchArgs& a, long cell, const Blk& b, const int* ist,
                         const double* rst, int ierr) {
-  double* c = b.C + cell * NSPEC;
-  for (int i = 0; i < NVAR; ++i) c[i] = w.ld(W_Y, i);
-  for (int k = 0; k < 8; ++k) b.ISTAT[cell * 8 + k] = ist[k];
-  for (int k = 0; k < 3; ++k) b.RSTAT[cell * 3 + k] = rst[k];
-  b.IERR[cell] = ierr;
+  double* c = a.C + cell * NSPEC;
+  for (int i = 0; i < NVAR; ++i) c[i] = b.ld(O_Y + i * 512);
+  for (int k = 0; k < 8; ++k) a.ISTAT[cell * 8 + k] = ist[k];
+  for (int k = 0; k < 3; ++k) a.RSTAT[cell * 3 + k] = rst[k];
+  a.IERR[cell] = ierr;
 }
 
 // Fcn0 <- Fun(Y); for non-autonomous problems also the dFdT replica
 // (gckpp_BatchIntegrator.F90:549-561).
-__device__ void step_functions(const Ws& w, const Opts& o, double T, int* ist) {
-  mech_fun(w.arr(W_Y), w.arr(W_FIX), w.arr(W_RCT), w.arr(W_A), w.arr(W_FCN0), w.S, w.slot);
+__device__ void step_functions(const Blk& b, const Opts& o, double T, int* ist) {
+  fun(b, false, O_FCN0);
   ist[0] += 1;
   if (!o.autonomous) {
     const double Delta = sqrt(ROUNDOFF) * dmax(1.0e-6, fabs(T));
-    mech_fun(w.arr(W_Y), w.arr(W_FIX), w.arr(W_RCT), w.arr(W_A), w.arr(W_DFDT), w.S, w.slot);
+    fun(b, false, O_DFDT);
     ist[0] += 1;
     for (int k = 0; k < NVAR; ++k)
-      w.st(W_DFDT, k, (w.ld(W_DFDT, k) - w.ld(W_FCN0, k)) / Delta);
+      b.st(O_DFDT + k * 512, (b.ld(O_DFDT + k * 512) - b.ld(O_FCN0 + k * 512)) / Delta);
   }
 }
 
 // The four stages after a successful factorisation, then Ynew, Yerr and the
 // error norm (gckpp_BatchIntegrator.F90:594-665).  Returns Err.
-__device__ double stages_and_error(const Ws& w, const Opts& o, double Dir, double H,
+__device__ double stages_and_error(const Blk& b, const Opts& o, double Dir, double H,
                                    const double* atol, const double* rtol, int* ist) {
-  double* const K = w.arr(W_K);
-  const long KS = (long)NVAR * w.S;  // stride between stage vectors
   for (int s = 0; s < 4; ++s) {
-    double* Ks = K + s * KS;
+    const int Ks = O_K + s * O_VEC;
     if (s == 0 || s == 1) {
       // stage 1: Fcn = Fcn0; stage 2 has ros_NewF = .FALSE. so Fcn is still Fcn0
-      for (int k = 0; k < NVAR; ++k) w.st(W_K + s * NVAR, k, w.ld(W_FCN0, k));
+      for (int k = 0; k < NVAR; ++k) b.st(Ks + k * 512, b.ld(O_FCN0 + k * 512));
     } else {
       for (int k = 0; k < NVAR; ++k) {
-        double y = w.ld(W_Y, k);
-        for (int j = 0; j < s; ++j) {
-          const double aa = ros_A[s * (s - 1) / 2 + j];
-          y = y + aa * w.ld(W_K + j * NVAR, k);
-        }
-        w.st(W_YNEW, k, y);
+        double y = b.ld(O_Y + k * 512);
+        for (int j = 0; j < s; ++j) y = y + ros_A[s * (s - 1) / 2 + j] * b.ld(O_K + j * O_VEC + k * 512);
+        b.st(O_YNEW + k * 512, y);
       }
-      mech_fun(w.arr(W_YNEW), w.arr(W_FIX), w.arr(W_RCT), w.arr(W_A), Ks, w.S, w.slot);
+      fun(b, true, Ks);
       ist[0] += 1;
     }
     for (int j = 0; j < s; ++j) {
       const double HC = ros_C[s * (s - 1) / 2 + j] / (Dir * H);
       for (int k = 0; k < NVAR; ++k)
-        w.st(W_K + s * NVAR, k, w.ld(W_K + s * NVAR, k) + HC * w.ld(W_K + j * NVAR, k));
+        b.st(Ks + k * 512, b.ld(Ks + k * 512) + HC * b.ld(O_K + j * O_VEC + k * 512));
     }
     if (!o.autonomous && ros_Gamma[s] != 0.0) {
       const double HG = (Dir * H) * ros_Gamma[s];
       for (int k = 0; k < NVAR; ++k)
-        w.st(W_K + s * NVAR, k, w.ld(W_K + s * NVAR, k) + HG * w.ld(W_DFDT, k));
+        b.st(Ks + k * 512, b.ld(Ks + k * 512) + HG * b.ld(O_DFDT + k * 512));
     }
-    mech_solve(w.arr(W_LU), Ks, w.S, w.slot);
+    solve(b, s);
     ist[6] += 1;
   }
   double Err = 0.0;
   for (int k = 0; k < NVAR; ++k) {
-    const double y = w.ld(W_Y, k);
+    const double y = b.ld(O_Y + k * 512);
     double yn = y, ye = 0.0;
-    for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * w.ld(W_K + j * NVAR, k);
-    for (int j = 0; j < 4; ++j) ye = ye + ros_E[j] * w.ld(W_K + j * NVAR, k);
-    w.st(W_YNEW, k, yn);
+    for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * b.ld(O_K + j * O_VEC + k * 512);
+    for (int j = 0; j < 4; ++j) ye = ye + ros_E[j] * b.ld(O_K + j * O_VEC + k * 512);
+    b.st(O_YNEW + k * 512, yn);
     const double Ymax = dmax(fabs(y), fabs(yn));
     const double Scal = o.vector_tol ? atol[k] + rtol[k] * Ymax : atol[0] + rtol[0] * Ymax;
     const double q = ye / Scal;
@@ -206,11 +409,11 @@ __device__ double stages_and_error(const Ws& w, const Opts& o, double Dir, doubl
   return dmax(Err, 1.0e-10);
 }
 
-__device__ void accept_y(const Ws& w, const Opts& o) {
+__device__ void accept_y(const Blk& b, const Opts& o) {
   if (o.clamp) {
-    for (int k = 0; k < NVAR; ++k) w.st(W_Y, k, dmax(w.ld(W_YNEW, k), 0.0));
+    for (int k = 0; k < NVAR; ++k) b.st(O_Y + k * 512, dmax(b.ld(O_YNEW + k * 512), 0.0));
   } else {
-    for (int k = 0; k < NVAR; ++k) w.st(W_Y, k, w.ld(W_YNEW, k));
+    for (int k = 0; k < NVAR; ++k) b.st(O_Y + k * 512, b.ld(O_YNEW + k * 512));
   }
 }
 
@@ -218,10 +421,8 @@ __device__ __forceinline__ bool time_left(int dir, double T, double Tend) {
   return (dir > 0 && ((T - Tend) + ROUNDOFF <= 0.0)) || (dir < 0 && ((Tend - T) + ROUNDOFF <= 0.0));
 }
 
-// ---------------------------------------------------------------------------
 // plain: Integrate_Cell_S per thread, with its own nested loops.
-// ---------------------------------------------------------------------------
-__device__ int integrate_cell(const Ws& w, const Opts& o, double Tstart, double Tend,
+__device__ int integrate_cell(const Blk& b, const Opts& o, double Tstart, double Tend,
                               const double* atol, const double* rtol, int* ist, double* rst) {
   int Nsteps = 0;
   double T = Tstart;
@@ -235,14 +436,13 @@ __device__ int integrate_cell(const Ws& w, const Opts& o, double Tstart, double 
     if (Nsteps > o.max_steps) return -6;
     if (((T + 0.1 * H) == T) || (H <= ROUNDOFF)) return -7;
     H = dmin(H, fabs(Tend - T));
-    step_functions(w, o, T, ist);
+    step_functions(b, o, T, ist);
     ist[1] += 1;  // Jac_SP of this step (evaluated in each attempt below)
-    for (;;) {  // UntilAccepted
+    for (;;) {    // UntilAccepted
       int ncons = 0;
       for (;;) {  // singular retries
         const double ghinv = 1.0 / ((Dir * H) * ros_Gamma[0]);
-        const int sing = mech_jac_prep(w.arr(W_Y), w.arr(W_FIX), w.arr(W_RCT), w.arr(W_LU),
-                                       ghinv, w.S, w.slot);
+        const int sing = jac_lu(b, ghinv);
         ist[5] += 1;
         if (!sing) break;
         ist[7] += 1;
@@ -250,15 +450,14 @@ __device__ int integrate_cell(const Ws& w, const Opts& o, double Tstart, double 
         if (ncons <= 5) H = H * 0.5;
         else return -8;
       }
-      mech_decomp(w.arr(W_LU), w.S, w.slot);
-      const double Err = stages_and_error(w, o, Dir, H, atol, rtol, ist);
+      const double Err = stages_and_error(b, o, Dir, H, atol, rtol, ist);
       const double Fac = dmin(o.FacMax, dmax(o.FacMin, o.FacSafe / gc_pow_third(Err)));
       double Hnew = H * Fac;
       Nsteps += 1;
       ist[2] += 1;
       if ((Err <= 1.0) || (H <= o.Hmin)) {
         ist[3] += 1;
-        accept_y(w, o);
+        accept_y(b, o);
         T = T + Dir * H;
         Hnew = dmax(o.Hmin, dmin(Hnew, o.Hmax));
         if (rejLast) Hnew = dmin(Hnew, H);
@@ -281,41 +480,38 @@ __device__ int integrate_cell(const Ws& w, const Opts& o, double Tstart, double 
   return 1;
 }
 
+enum : int { PH_FREE = 0, PH_STEP = 1, PH_RETRY = 2, PH_SING = 3 };
+
 }  // namespace
 
 extern "C" __global__ void
-__attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
-GC_SFX(gc_rodas3_plain)(GcBatchArgs b, double* ws, unsigned S) {
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_plain)(GcBatchArgs a, double* ws) {
+  extern __shared__ double lds[];
   const unsigned ic = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ic >= (unsigned)b.nCell) return;
-  const Ws w{ws, S, ic};
-  const long cell = (long)b.Perm[ic] - 1;
+  const int lane = threadIdx.x & 63;
+  const Blk b = make_blk(ws, ic >> 6, lane, lds + (threadIdx.x >> 6) * ((GC_SEG + 1) * 64));
+  if (ic >= (unsigned)a.nCell) return;
+  const long cell = (long)a.Perm[ic] - 1;
   int ist[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double rst[3] = {0.0, 0.0, 0.0};
-  gather(b, cell, w);
+  gather(a, cell, b);
   Opts o;
-  int ierr = decode_opts(b.ICNTRL + cell * 20, b.RCNTRL + cell * 20, b.Tstart, b.Tend, o);
-  if (ierr == 0) ierr = integrate_cell(w, o, b.Tstart, b.Tend, b.ATOL, b.RTOL, ist, rst);
-  scatter(b, cell, w, ist, rst, ierr);
-}
-
-// ---------------------------------------------------------------------------
-// persistent: work queue + flattened per-lane state machine.
-// ---------------------------------------------------------------------------
-namespace {
-enum : int { PH_FREE = 0, PH_STEP = 1, PH_RETRY = 2, PH_SING = 3 };
+  int ierr = decode_opts(a.ICNTRL + cell * 20, a.RCNTRL + cell * 20, a.Tstart, a.Tend, o);
+  if (ierr == 0) ierr = integrate_cell(b, o, a.Tstart, a.Tend, a.ATOL, a.RTOL, ist, rst);
+  scatter(a, cell, b, ist, rst, ierr);
 }
 
 extern "C" __global__ void
-__attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
-GC_SFX(gc_rodas3_persistent)(GcBatchArgs b, double* ws, unsigned S, unsigned* queue,
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* queue,
                              int cells_per_wave, int grain) {
+  extern __shared__ double lds[];
   const unsigned slot = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const Ws w{ws, S, slot};
+  const Blk b = make_blk(ws, slot >> 6, lane, lds + (threadIdx.x >> 6) * ((GC_SEG + 1) * 64));
   const bool lane_on = (lane < cells_per_wave) && (slot < S);
 
-  // per-lane state
   int phase = PH_FREE;
   long cell = -1;
   Opts o;
@@ -325,8 +521,7 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs b, double* ws, unsigned S, unsigned* qu
   int dir = 1, Nsteps = 0, ncons = 0;
   bool rejLast = false, rejMore = false;
   bool exhausted = !lane_on;
-  // wave-uniform reservation of queue positions [rsv, rsv_end)
-  unsigned rsv = 0, rsv_end = 0;
+  unsigned rsv = 0, rsv_end = 0;  // wave-uniform reservation [rsv, rsv_end)
 
   for (;;) {
     // ---- refill free lanes from the queue (cost order) -------------------
@@ -339,8 +534,9 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs b, double* ws, unsigned S, unsigned* qu
         unsigned base = 0;
         const unsigned g = (unsigned)(grain > 0 ? grain : 1);
         const unsigned take = (g > cnt) ? g : cnt;
-        if (lane == __ffsll((long long)mask) - 1) base = atomicAdd(queue, take);
-        base = __shfl(base, __ffsll((long long)mask) - 1);
+        const int leader = __ffsll((long long)mask) - 1;
+        if (lane == leader) base = atomicAdd(queue, take);
+        base = __shfl(base, leader);
         rsv = base;
         rsv_end = base + take;
       }
@@ -348,21 +544,21 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs b, double* ws, unsigned S, unsigned* qu
       const unsigned rank = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
       if (need && rank < avail) {
         const unsigned q = rsv + rank;
-        if (q < (unsigned)b.nCell) {
-          cell = (long)b.Perm[q] - 1;
+        if (q < (unsigned)a.nCell) {
+          cell = (long)a.Perm[q] - 1;
           for (int k = 0; k < 8; ++k) ist[k] = 0;
           for (int k = 0; k < 3; ++k) rst[k] = 0.0;
-          gather(b, cell, w);
-          const int e = decode_opts(b.ICNTRL + cell * 20, b.RCNTRL + cell * 20, b.Tstart, b.Tend, o);
+          gather(a, cell, b);
+          const int e = decode_opts(a.ICNTRL + cell * 20, a.RCNTRL + cell * 20, a.Tstart, a.Tend, o);
           if (e != 0) {
-            scatter(b, cell, w, ist, rst, e);
-            cell = -1;  // stays free: take another
+            scatter(a, cell, b, ist, rst, e);
+            cell = -1;  // stays free: takes another cell
           } else {
             Nsteps = 0;
-            T = b.Tstart;
+            T = a.Tstart;
             H = dmin(dmax(fabs(o.Hmin), fabs(o.Hstart)), fabs(o.Hmax));
             if (fabs(H) <= 10.0 * ROUNDOFF) H = DELTAMIN;
-            dir = (b.Tend >= b.Tstart) ? +1 : -1;
+            dir = (a.Tend >= a.Tstart) ? +1 : -1;
             Dir = (double)dir;
             H = Dir * H;
             rejLast = rejMore = false;
@@ -373,29 +569,26 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs b, double* ws, unsigned S, unsigned* qu
         }
       }
       rsv += (cnt < avail) ? cnt : avail;
-      if (rsv >= (unsigned)b.nCell && rsv >= rsv_end) {
-        // queue drained for this wave: lanes still free are done
-        if (phase == PH_FREE) exhausted = true;
-      }
+      if (rsv >= (unsigned)a.nCell && rsv >= rsv_end && phase == PH_FREE) exhausted = true;
     }
     if (!__any(phase != PH_FREE)) break;
 
     // ---- top of TimeLoop for lanes starting a step -----------------------
-    int ierr_now = 0;
     if (phase == PH_STEP) {
-      if (!time_left(dir, T, b.Tend)) ierr_now = 1;
+      int ierr_now = 0;
+      if (!time_left(dir, T, a.Tend)) ierr_now = 1;
       else if (Nsteps > o.max_steps) ierr_now = -6;
       else if (((T + 0.1 * H) == T) || (H <= ROUNDOFF)) ierr_now = -7;
       if (ierr_now != 0) {
-        scatter(b, cell, w, ist, rst, ierr_now);
+        scatter(a, cell, b, ist, rst, ierr_now);
         phase = PH_FREE;
         cell = -1;
       } else {
-        H = dmin(H, fabs(b.Tend - T));
+        H = dmin(H, fabs(a.Tend - T));
       }
     }
     if (phase == PH_STEP) {
-      step_functions(w, o, T, ist);
+      step_functions(b, o, T, ist);
       ist[1] += 1;
     }
     if (phase == PH_STEP || phase == PH_RETRY) ncons = 0;
@@ -407,8 +600,7 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs b, double* ws, unsigned S, unsigned* qu
     bool factored = false;
     if (phase != PH_FREE) {
       const double ghinv = 1.0 / ((Dir * H) * ros_Gamma[0]);
-      const int sing = mech_jac_prep(w.arr(W_Y), w.arr(W_FIX), w.arr(W_RCT), w.arr(W_LU),
-                                     ghinv, w.S, w.slot);
+      const int sing = jac_lu(b, ghinv);
       ist[5] += 1;
       if (sing) {
         ist[7] += 1;
@@ -417,7 +609,7 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs b, double* ws, unsigned S, unsigned* qu
           H = H * 0.5;
           phase = PH_SING;
         } else {
-          scatter(b, cell, w, ist, rst, -8);
+          scatter(a, cell, b, ist, rst, -8);
           phase = PH_FREE;
           cell = -1;
         }
@@ -426,15 +618,14 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs b, double* ws, unsigned S, unsigned* qu
       }
     }
     if (factored) {
-      mech_decomp(w.arr(W_LU), w.S, w.slot);
-      const double Err = stages_and_error(w, o, Dir, H, b.ATOL, b.RTOL, ist);
+      const double Err = stages_and_error(b, o, Dir, H, a.ATOL, a.RTOL, ist);
       const double Fac = dmin(o.FacMax, dmax(o.FacMin, o.FacSafe / gc_pow_third(Err)));
       double Hnew = H * Fac;
       Nsteps += 1;
       ist[2] += 1;
       if ((Err <= 1.0) || (H <= o.Hmin)) {
         ist[3] += 1;
-        accept_y(w, o);
+        accept_y(b, o);
         T = T + Dir * H;
         Hnew = dmax(o.Hmin, dmin(Hnew, o.Hmax));
         if (rejLast) Hnew = dmin(Hnew, H);
@@ -458,30 +649,35 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs b, double* ws, unsigned S, unsigned* qu
 }
 
 // ---------------------------------------------------------------------------
-// host launch wrappers of this register-budget variant
+// host launch wrappers of this variant
 // ---------------------------------------------------------------------------
-extern "C" hipError_t GC_SFX(gc_launch_plain)(const GcBatchArgs* b, double* ws, unsigned S,
-                                               int block, hipStream_t st) {
-  if (b->nCell <= 0) return hipSuccess;
-  const unsigned grid = ((unsigned)b->nCell + (unsigned)block - 1u) / (unsigned)block;
-  hipLaunchKernelGGL(GC_SFX(gc_rodas3_plain), dim3(grid), dim3(block), 0, st, *b, ws, S);
+extern "C" hipError_t GC_SFX(gc_launch_plain)(const GcBatchArgs* a, double* ws, int block,
+                                               hipStream_t st) {
+  if (a->nCell <= 0) return hipSuccess;
+  const unsigned grid = ((unsigned)a->nCell + (unsigned)block - 1u) / (unsigned)block;
+  const size_t lds = (size_t)(block / 64) * (GC_SEG + 1) * 64 * sizeof(double);
+  hipLaunchKernelGGL(GC_SFX(gc_rodas3_plain), dim3(grid), dim3(block), lds, st, *a, ws);
   return hipGetLastError();
 }
 
-extern "C" hipError_t GC_SFX(gc_launch_persistent)(const GcBatchArgs* b, double* ws, unsigned S,
+extern "C" hipError_t GC_SFX(gc_launch_persistent)(const GcBatchArgs* a, double* ws, unsigned S,
                                                     unsigned* queue, int block,
                                                     int cells_per_wave, int grain,
                                                     hipStream_t st) {
-  if (b->nCell <= 0) return hipSuccess;
+  if (a->nCell <= 0) return hipSuccess;
   const unsigned grid = S / (unsigned)block;
-  hipLaunchKernelGGL(GC_SFX(gc_rodas3_persistent), dim3(grid), dim3(block), 0, st, *b, ws, S,
+  const size_t lds = (size_t)(block / 64) * (GC_SEG + 1) * 64 * sizeof(double);
+  hipLaunchKernelGGL(GC_SFX(gc_rodas3_persistent), dim3(grid), dim3(block), lds, st, *a, ws, S,
                      queue, cells_per_wave, grain);
   return hipGetLastError();
 }
 
-extern "C" void GC_SFX(gc_mech_dims)(GcMechDims* d) {
+extern "C" void GC_SFX(gc_variant_info)(GcVariantInfo* d) {
   d->nvar = NVAR;
   d->nfix = NFIX;
   d->nreact = NREACT;
-  d->nnz = NNZ;
+  d->nnz = MECH_NNZ;
+  d->ws_doubles_per_slot = E_TOTAL;
+  d->waves_per_eu = GC_WPE;
+  d->seg = GC_SEG;
 }

@@ -3,11 +3,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-// Doubles of slot workspace per cell slot: Y, Ynew, Fcn0, 4 stage vectors,
-// dFdT (8*NVAR); RCONST copy and the rate scratch (2*NREACT); FIX; LU.
-#define GC_WS_DOUBLES_PER_SLOT(nvar, nfix, nreact, nnz) \
-  (8L * (nvar) + 2L * (nreact) + (long)(nfix) + (long)(nnz))
-
 // Device pointers, caller (Fortran) layout: C(NSPEC,nCell), RCONST(NREACT,nCell),
 // ICNTRL(20,nCell), RCNTRL(20,nCell), Perm(nCell) 1-based, ISTAT(8,nCell),
 // RSTAT(3,nCell), IERR(nCell); ATOL/RTOL(NVAR) shared by all cells.
@@ -26,18 +21,23 @@ struct GcBatchArgs {
   int* IERR;
 };
 
-struct GcMechDims {
+// What a compiled variant is: mechanism dimensions, doubles of workspace per
+// cell slot (the workspace is slots/64 wave blocks of 64*ws_doubles_per_slot
+// doubles), its occupancy target and its LU row-segment size in LDS.
+struct GcVariantInfo {
   int nvar, nfix, nreact, nnz;
+  long ws_doubles_per_slot;
+  int waves_per_eu, seg;
 };
 
-#define GC_DECLARE_VARIANT(W)                                                         \
-  extern "C" hipError_t gc_launch_plain_w##W(const GcBatchArgs* b, double* ws,       \
-                                              unsigned S, int block, hipStream_t st); \
-  extern "C" hipError_t gc_launch_persistent_w##W(const GcBatchArgs* b, double* ws,  \
-                                                   unsigned S, unsigned* queue,      \
-                                                   int block, int cells_per_wave,    \
-                                                   int grain, hipStream_t st);       \
-  extern "C" void gc_mech_dims_w##W(GcMechDims* d);
+#define GC_DECLARE_VARIANT(V)                                                          \
+  extern "C" hipError_t gc_launch_plain_v##V(const GcBatchArgs* a, double* ws,        \
+                                              int block, hipStream_t st);             \
+  extern "C" hipError_t gc_launch_persistent_v##V(const GcBatchArgs* a, double* ws,   \
+                                                   unsigned S, unsigned* queue,       \
+                                                   int block, int cells_per_wave,     \
+                                                   int grain, hipStream_t st);        \
+  extern "C" void gc_variant_info_v##V(GcVariantInfo* d);
 
 GC_DECLARE_VARIANT(1)
 GC_DECLARE_VARIANT(2)

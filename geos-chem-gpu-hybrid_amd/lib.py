@@ -73,11 +73,13 @@ def mech_info(lib):
 def get_tuning(lib) -> dict:
     v = [ctypes.c_int() for _ in range(5)]
     lib.gc_amd_get_tuning(*(ctypes.byref(x) for x in v))
-    keys = ("block", "blocks_per_cu", "cells_per_wave", "grain", "regs")
+    keys = ("block", "blocks_per_cu", "cells_per_wave", "grain", "variant")
     return dict(zip(keys, (x.value for x in v)))
 
 
-def set_tuning(lib, block=0, blocks_per_cu=0, cells_per_wave=0, grain=0, regs=0):
-    rc = lib.gc_amd_set_tuning(block, blocks_per_cu, cells_per_wave, grain, regs)
+def set_tuning(lib, block=0, blocks_per_cu=-1, cells_per_wave=0, grain=0, variant=0):
+    """Set launch parameters; 0 (-1 for blocks_per_cu) keeps the current value."""
+    rc = lib.gc_amd_set_tuning(block, blocks_per_cu, cells_per_wave, grain, variant)
     if rc != 0:
-        raise ValueError(f"gc_amd_set_tuning rejected {block, blocks_per_cu, cells_per_wave, grain, regs}")
+        raise ValueError(f"gc_amd_set_tuning rejected "
+                         f"{block, blocks_per_cu, cells_per_wave, grain, variant}")

@@ -15,7 +15,7 @@ from . import PKG_DIR
 from . import lib as _lib
 
 TUNE_DIR = PKG_DIR / "tuning"
-LAUNCH_KEYS = ("block", "blocks_per_cu", "cells_per_wave", "grain", "regs")
+LAUNCH_KEYS = ("block", "blocks_per_cu", "cells_per_wave", "grain", "variant")
 
 
 def load(mech: str = "fullchem") -> dict:

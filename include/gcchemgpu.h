@@ -81,8 +81,9 @@ void gc_gpu_selftest(int *rc, int *nsteps, double *tTotal);
 #define GC_AMD_KERNEL_PERSISTENT 1 /* work queue + flattened state machine  */
 
 /* Batch solve with every array already in device memory (same layouts and
- * meaning as gc_gpu_integrate_batch).  `stream` is a hipStream_t (NULL: the
- * library's own stream of the current device).  Returns 0 on success or a
+ * meaning as gc_gpu_integrate_batch).  `stream` is the caller's hipStream_t
+ * (NULL: the null stream): the solve starts after all work already queued on
+ * it, on the library's own stream of the current device.  Returns 0 on success or a
  * negative code: -1 bad argument (nCell < 0, Perm entry outside 1..nCell,
  * unknown kernel), -2 device memory, -3 launch or runtime error (details in
  * gc_amd_last_error).  *tKernel (may be NULL) receives the device time of the
@@ -104,13 +105,13 @@ int gc_amd_integrate_batch_device(int kernel, int chunk, int nCell,
  *   blocks_per_cu   resident work-groups per compute unit (grid = CUs * this)
  *   cells_per_wave  lanes of each 64-wide wave that take cells (16..64)
  *   grain           queue positions a wave claims per atomic (1..4096)
- *   regs            register-budget variant: 1 = 512 VGPR/lane (1 wave per
- *                   SIMD), 2 = 256 VGPR/lane (2 waves per SIMD)
+ *   variant         LU row-segment variant: 1 = 32-entry segments in LDS
+ *                   (2 waves per SIMD), 2 = 16-entry segments (4 waves per SIMD)
  * Returns 0, or -1 if a value is out of range (nothing changed). */
 int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave,
-                      int grain, int regs);
+                      int grain, int variant);
 void gc_amd_get_tuning(int *block, int *blocks_per_cu, int *cells_per_wave,
-                       int *grain, int *regs);
+                       int *grain, int *variant);
 
 /* Dimensions of the mechanism compiled into this library. */
 void gc_amd_mech_info(int *nvar, int *nfix, int *nreact, int *lu_nonzero);

@@ -133,14 +133,35 @@ def test_permutation_and_chunks(cfg0, ref0, mech, kernel):
     _assert_bitexact(_gpu(cfg0, kernel, chunk=700), ref0, mech.nvar)
 
 
+@pytest.mark.parametrize("kernel", ["plain", "persistent"])
+def test_solve_ordered_after_callers_stream(cfg0, ref0, mech, kernel):
+    """The solve reads the inputs as the caller's stream leaves them: work
+    queued on torch's stream right before the call (a slow kernel, then the
+    copy of the real concentrations over zeros) completes before the library
+    reads C.  Regression test for a solve that started on the library's own
+    stream while the copy was still queued."""
+    b = solver.Batch.from_numpy(cfg0)
+    good = b.C.clone()
+    b.C.zero_()
+    torch.cuda.synchronize()
+    torch.cuda._sleep(100_000_000)  # keeps the caller's stream busy for tens of ms
+    b.C.copy_(good)
+    r = solver.solve(b, kernel=kernel, inplace=True)
+    torch.cuda.synchronize()
+    g = (r.C.cpu().numpy(), r.ISTAT.cpu().numpy(), r.RSTAT.cpu().numpy(), r.IERR.cpu().numpy())
+    _assert_bitexact(g, ref0, mech.nvar)
+
+
 def test_tuning_variants_identical(cfg0, ref0, mech):
     lib = L.load()
     old = L.get_tuning(lib)
     try:
-        for t in ({"regs": 2}, {"cells_per_wave": 32, "grain": 1}, {"block": 64, "blocks_per_cu": 4},
-                  {"cells_per_wave": 16, "grain": 4096, "regs": 1}):
+        for t in ({"variant": 1}, {"variant": 2, "cells_per_wave": 32, "grain": 1},
+                  {"block": 64, "blocks_per_cu": 4},
+                  {"cells_per_wave": 16, "grain": 4096, "variant": 1, "blocks_per_cu": 0}):
             L.set_tuning(lib, **t)
             _assert_bitexact(_gpu(cfg0, "persistent"), ref0, mech.nvar)
+            print(f"tuning {t} ok", flush=True)
     finally:
         L.set_tuning(lib, **old)
 
@@ -206,18 +227,19 @@ def stress():
 @pytest.mark.parametrize("kernel", ["plain", "persistent"])
 def test_stress_mechanism_bitexact(stress, kernel):
     d = inputs.config4(stress, n=256, seed=4)
+    d["Tend"] = 20.0
     ref = _run_oracle(Oracle(stress.ir, pow_mode="cr"), d)
     _assert_bitexact(_gpu(d, kernel, mech="stress500"), ref, stress.nvar)
 
 
 def test_stress_ragged_batches(stress):
     """Independent batches of different sizes in turn (the config-4 ragged
-    stress, scaled down to a 60 s step): each matches the oracle on a
+    stress, scaled down to a 20 s step): each matches the oracle on a
     sample of its cells."""
     o = Oracle(stress.ir, pow_mode="cr")
     for i, n in enumerate((1, 100, 1000)):
         d = inputs.config4(stress, n=n, seed=40 + i)
-        d["Tend"] = 60.0
+        d["Tend"] = 20.0
         g = _gpu(d, "persistent", mech="stress500")
         idx = np.unique(np.linspace(0, n - 1, min(n, 16)).astype(int))
         ref = _run_oracle(o, inputs.subset(d, idx))

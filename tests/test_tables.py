@@ -1,0 +1,254 @@
+"""The generated operand tables, interpreted on the CPU, against the oracle.
+
+``TableMachine`` walks the int32 tables of codegen/gen_tables.py with exactly
+the loops of the device routines in csrc/gc_rodas3.hip (fixed-shape product
+groups, folded-sign sums, the fused Jacobian/LU ops with an LDS row segment,
+the padded solve rows), vectorised over lanes with numpy (no fused multiply-add,
+IEEE double, same order of operations).  Every result must equal the oracle's
+bit for bit, so a table generator mistake (a wrong offset, a dropped term, a
+lost sign, a padding entry that is not a no-op) fails here without a GPU.
+"""
+import importlib.util
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import PKG
+
+
+def _gen_tables():
+    spec = importlib.util.spec_from_file_location("gen_tables", PKG / "codegen" / "gen_tables.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+GT = _gen_tables()
+
+
+def _coef(lo, hi):
+    return struct.unpack("<d", struct.pack("<ii", lo, hi))[0]
+
+
+class TableMachine:
+    """CPU interpreter of one mechanism's tables over ``lanes`` cells."""
+
+    def __init__(self, ir, lanes, seg):
+        self.t = GT.Tables(ir)
+        self.E = self.t.E
+        self.ir = ir
+        self.seg = seg
+        self.prod = {"rate_y": self.t.rates("Y"), "rate_yn": self.t.rates("YNEW"),
+                     "dprod_y": self.t.dprods("Y"), "dprod_yn": self.t.dprods("YNEW"),
+                     "bjac": self.t.bjac()}
+        self.pd = {"y": self.t.pd("Y"), "yn": self.t.pd("YNEW")}
+        self.lu_ops = self.t.lu(seg)
+        self.fwd = [self.t.fwd("K", s) for s in range(4)]
+        self.bwd = [self.t.bwd("K", s) for s in range(4)]
+        self.lits = np.array(self.t.lits or [0.0])
+        self.ws = np.full((self.E["TOTAL"], lanes), np.nan)
+        self.ws[self.E["ZERO"]] = 0.0
+        self.lds = np.full((seg + 1, lanes), np.nan)
+
+    # workspace access by byte offset e*512 (a lane's column)
+    def ld(self, soff):
+        assert soff % 512 == 0
+        return self.ws[soff // 512]
+
+    def st(self, soff, v):
+        self.ws[soff // 512] = v
+
+    def products(self, name):
+        words, counts = self.prod[name]
+        r = 0
+        for shape, n in enumerate(counts):
+            nf, lit = shape // 2 + 1, shape % 2
+            for _ in range(n):
+                rec = words[8 * r:8 * r + 8]
+                v = self.ld(rec[2]).copy()
+                if lit:
+                    v = self.lits[rec[1]] * v
+                for f in range(1, nf):
+                    v = v * self.ld(rec[2 + f])
+                self.st(rec[0], v)
+                r += 1
+        assert 8 * r == len(words)
+
+    def sum_terms(self, w, p, n):
+        acc = _coef(w[p + 2], w[p + 3]) * self.ld(w[p])
+        for k in range(1, n):
+            q = p + 4 * k
+            acc = acc + _coef(w[q + 2], w[q + 3]) * self.ld(w[q])
+        return acc, p + 4 * n
+
+    def fun(self, ynew, out):
+        sfx = "yn" if ynew else "y"
+        self.products("rate_" + sfx)
+        self.products("dprod_" + sfx)
+        w, p = self.pd[sfx], 0
+        for _ in range(self.ir["nvar"]):
+            nP, nD, vo, oo = w[p:p + 4]
+            p += 4
+            P = D = 0.0
+            if nP:
+                P, p = self.sum_terms(w, p, nP)
+            if nD:
+                D, p = self.sum_terms(w, p, nD)
+            self.st(out + oo, P - D * self.ld(vo))
+        assert p == len(w)
+
+    def jac_lu(self, ghinv):
+        self.products("bjac")
+        w, p, sing = self.lu_ops, 0, np.zeros(self.ws.shape[1], bool)
+        lds = lambda off: self.lds[off // 512]
+        while True:
+            code = w[p]
+            if code == 0:
+                break
+            if code == 1:
+                slot, nt, isdiag = w[p + 1:p + 4]
+                p += 4
+                if nt:
+                    v, p = self.sum_terms(w, p, nt)
+                else:
+                    v = np.zeros(self.ws.shape[1])
+                g = -v
+                if isdiag:
+                    g = g + ghinv
+                    sing |= np.abs(g) < 2.2250738585072014e-308
+                self.lds[slot // 512] = g
+            elif code in (2, 3):
+                if code == 2:
+                    sj = w[p + 1]
+                    a = -lds(sj) / self.ld(w[p + 2])
+                    self.lds[sj // 512] = -a
+                else:
+                    a = -self.ld(w[p + 1])
+                n = w[p + 3]
+                assert n % 4 == 0
+                p += 4
+                for _ in range(n):
+                    t, u = w[p], w[p + 1]
+                    self.lds[t // 512] = lds(t) + a * self.ld(u)
+                    p += 2
+            else:
+                assert code == 4
+                n = w[p + 1]
+                assert n % 2 == 0
+                p += 4
+                for _ in range(n):
+                    self.st(w[p + 1], lds(w[p]).copy())
+                    p += 2
+        return sing
+
+    def solve_rows(self, rows, div):
+        w, nrows = rows
+        p = 0
+        for _ in range(nrows):
+            n, xi, dg = w[p:p + 3]
+            assert n % 4 == 0
+            p += 4
+            x = self.ld(xi).copy()
+            for _ in range(n):
+                x = x - self.ld(w[p]) * self.ld(w[p + 1])
+                p += 2
+            if div:
+                x = x / self.ld(dg)
+            self.st(xi, x)
+        assert p == len(w)
+
+    def solve(self, s):
+        self.solve_rows(self.fwd[s], False)
+        self.solve_rows(self.bwd[s], True)
+
+    def vec(self, name, k=0):
+        n = self.ir["nvar"]
+        e = self.E[name] + k * n
+        return self.ws[e:e + n]
+
+
+def _cells(mech, lanes, seed):
+    rng = np.random.default_rng(seed)
+    nvar, nfix, nreact = mech.nvar, mech.nfix, mech.nreact
+    V = 10.0 ** rng.uniform(-2, 10, (lanes, nvar))
+    V[:, ::17] = 0.0                      # exact zeros exercise the signed sums
+    F = 10.0 ** rng.uniform(15, 19, (lanes, nfix))
+    R = 10.0 ** rng.uniform(-20, -8, (lanes, nreact))
+    return V, F, R
+
+
+def _load(m, V, F, R):
+    E, n = m.E, V.shape[1]
+    m.ws[E["Y"]:E["Y"] + n] = V.T
+    m.ws[E["FIX"]:E["FIX"] + F.shape[1]] = F.T
+    m.ws[E["RCT"]:E["RCT"] + R.shape[1]] = R.T
+
+
+def _same(a, b):
+    """Bitwise equality of float arrays (NaN-free)."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+LANES = 4
+
+
+@pytest.mark.parametrize("seg", GT.SEGMENTS)
+def test_tables_match_oracle_fullchem(mech, oracle_cr, seg):
+    _check(mech, oracle_cr, seg, seed=seg)
+
+
+@pytest.mark.parametrize("name", ["stress500", "auto1"])
+def test_tables_match_oracle_synthetic(name):
+    from gcchem_amd import mechanism
+    from oracle import Oracle
+    m = mechanism.load(name)
+    _check(m, Oracle(m.ir, pow_mode="cr"), 16, seed=7)
+
+
+def _check(mech, orc, seg, seed):
+    ir = mech.ir
+    n = ir["nvar"]
+    V, F, R = _cells(mech, LANES, seed)
+    m = TableMachine(ir, LANES, seg)
+    _load(m, V, F, R)
+    # Fun at Y into FCN0, and at Ynew (= perturbed Y) into K_1
+    m.fun(False, m.E["FCN0"] * 512)
+    Vn = V * 1.25
+    m.ws[m.E["YNEW"]:m.E["YNEW"] + n] = Vn.T
+    m.fun(True, (m.E["K"] + n) * 512)
+    ghinv = np.array([1.0e3, 3.7, 0.05, 1.0e-6])[:LANES]
+    sing = m.jac_lu(ghinv)
+    lu_dev = m.ws[m.E["LU"]:m.E["LU"] + ir["nnz"]]
+    # right-hand sides for each stage, solved in place
+    rhs = np.stack([np.sin(np.arange(n) + s) * 10.0 ** (s - 2) for s in range(4)])
+    for s in range(4):
+        m.ws[m.E["K"] + s * n:m.E["K"] + (s + 1) * n] = rhs[s][:, None]
+        m.solve(s)
+    diag = np.asarray(ir["diag"])
+    for c in range(LANES):
+        vdot, *_ = orc.fun_split(V[c], F[c], R[c])
+        assert _same(m.vec("FCN0")[:, c], vdot)
+        jvs = orc.jac_sp(V[c], F[c], R[c])
+        g = -jvs
+        g[diag] = g[diag] + ghinv[c]
+        assert bool(sing[c]) == bool(np.any(np.abs(g[diag]) < 2.2250738585072014e-308))
+        lu, ier = orc.decomp(g)
+        assert ier == 0
+        assert _same(lu_dev[:, c], lu)
+        for s in range(4):
+            assert _same(m.vec("K", s)[:, c], orc.solve(lu, rhs[s]))
+    # Fun at Ynew was written into K_1 before the solves overwrote it: redo it
+    m.fun(True, (m.E["K"] + n) * 512)
+    for c in range(LANES):
+        vdot_n, *_ = orc.fun_split(Vn[c], F[c], R[c])
+        assert _same(m.vec("K", 1)[:, c], vdot_n)
+
+
+def test_padding_is_exact_noop():
+    """-0.0 - 0.0*0.0 keeps the sign, and an LDS padding update only touches
+    the spare slot."""
+    x = np.array([-0.0, 0.0, 1e-300, -3.5])
+    y = x - np.float64(0.0) * np.float64(0.0)
+    assert _same(y, x)
