@@ -47,8 +47,10 @@ def log(*a):
 
 
 def cpu_baseline(mech, d, perm, target_s=15.0):
-    """Oracle (libm pow) on every k-th cell of the launch order, sized to
-    about target_s seconds of wall time on all host cores."""
+    """Oracle (libm pow) on cells evenly spaced in the launch order, sized to
+    about target_s seconds of wall time on all host cores: a calibration run
+    of 16 cells per thread fixes the per-cell rate, the sample is scaled from
+    it."""
     sys.path.insert(0, str(REPO / "oracle"))
     import numpy as np
     from oracle import Oracle
@@ -56,18 +58,57 @@ def cpu_baseline(mech, d, perm, target_s=15.0):
     o = Oracle(mech.ir, pow_mode="libm")
     threads = min(os.cpu_count() or 1, 64)
     n_all = d["C"].shape[0]
-    cal = inputs.subset(d, perm[np.linspace(0, n_all - 1, 2 * threads).astype(int)] - 1)
-    t = o.integrate_batch(cal["Tstart"], cal["Tend"], cal["C"], cal["RCONST"], cal["ATOL"],
-                          cal["RTOL"], cal["ICNTRL"], cal["RCNTRL"], nthreads=threads)[4]
-    per_cell = t / cal["C"].shape[0]
-    n = int(max(2 * threads, min(n_all, target_s / max(per_cell, 1e-6))))
-    idx = perm[np.linspace(0, n_all - 1, n).astype(int)] - 1
-    s = inputs.subset(d, idx)
-    secs = o.integrate_batch(s["Tstart"], s["Tend"], s["C"], s["RCONST"], s["ATOL"], s["RTOL"],
-                             s["ICNTRL"], s["RCNTRL"], nthreads=threads)[4]
-    return {"value": n / secs, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"{n} cells evenly spaced in the launch order of the same 238,464-cell "
-                      f"workload, OpenMP schedule(dynamic,16), {secs:.1f} s wall"}
+
+    def run(n):
+        idx = perm[np.linspace(0, n_all - 1, n).astype(int)] - 1
+        s = inputs.subset(d, idx)
+        return o.integrate_batch(s["Tstart"], s["Tend"], s["C"], s["RCONST"], s["ATOL"],
+                                 s["RTOL"], s["ICNTRL"], s["RCNTRL"], nthreads=threads)[4]
+
+    n_cal = min(n_all, 16 * threads)
+    t_cal = run(n_cal)
+    n = int(min(n_all, max(n_cal, n_cal * target_s / max(t_cal, 1e-3))))
+    log(f"cpu baseline: calibration {n_cal} cells in {t_cal:.2f} s -> sample of {n} cells")
+    secs = run(n)
+    return {"value": round(n / secs, 1), "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"{n} cells evenly spaced in the launch order of the same {n_all}-cell "
+                      f"workload, OpenMP schedule(dynamic,16) on {threads} threads, "
+                      f"{secs:.1f} s wall"}
+
+
+def host_api_rate(lib, d, perm):
+    """One call of the reference entry point gc_gpu_integrate_batch on host
+    arrays (PCIe staging in and out included): (cells/s, tTotal, tKernel)."""
+    import ctypes
+    import numpy as np
+    n = d["C"].shape[0]
+    arr = {k: np.ascontiguousarray(d[k]).copy() for k in ("C", "RCONST", "ATOL", "RTOL",
+                                                          "ICNTRL", "RCNTRL")}
+    Perm = np.ascontiguousarray(perm, dtype=np.int32)
+    ISTAT = np.zeros((n, 8), np.int32)
+    RSTAT = np.zeros((n, 3))
+    IERR = np.zeros(n, np.int32)
+    tT, tK = ctypes.c_double(), ctypes.c_double()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib.gc_gpu_integrate_batch(n, float(d["Tstart"]), float(d["Tend"]), p(arr["C"]),
+                               p(arr["RCONST"]), p(arr["ATOL"]), p(arr["RTOL"]),
+                               p(arr["ICNTRL"]), p(arr["RCNTRL"]), p(Perm), p(ISTAT), p(RSTAT),
+                               p(IERR), ctypes.byref(tT), ctypes.byref(tK))
+    if not (IERR == 1).all():
+        raise RuntimeError(f"host entry point: {int((IERR != 1).sum())} cells failed")
+    return n / tT.value, tT.value, tK.value
+
+
+def copy_probe(torch, gib=1):
+    """One elementwise copy kernel that reads and writes `gib` GiB: its
+    FETCH_SIZE / WRITE_SIZE readings calibrate the counters of a profile."""
+    n = gib * (1 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"copy probe: elementwise copy of {gib} GiB (reads {gib} GiB, writes {gib} GiB)")
+    del src, dst
 
 
 def main():
@@ -77,6 +118,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=int, default=0, help="use the first N cells (debug)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB elementwise copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the batched Rodas3 solve has no cross-GPU "
@@ -89,6 +132,8 @@ def main():
     from gcchem_amd import inputs, lib as L, mechanism, solver, tuning
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        copy_probe(torch)
     mech = mechanism.load("fullchem")
     lib = L.load("fullchem")
     tune = tuning.apply(lib, "fullchem")
@@ -103,6 +148,7 @@ def main():
     # previous chemistry step -> cost order (untimed)
     r = solver.solve(b, kernel="persistent")
     torch.cuda.synchronize()
+    log(f"previous step (grid order): kernel {r.t_kernel * 1e3:.1f} ms")
     perm = inputs.cost_order(r.ISTAT[:, 2].cpu().numpy())
     b.Perm = torch.as_tensor(perm, device=b.C.device)
     out = None
@@ -114,9 +160,10 @@ def main():
         torch.cuda.synchronize()
         tk = []
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for k in range(steps):
             out = solver.solve(b, kernel=kernel, out=out)
             tk.append(out.t_kernel)
+            log(f"{kernel} step {k}: kernel {out.t_kernel * 1e3:.1f} ms")
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, float(np.mean(tk))
 
@@ -126,6 +173,9 @@ def main():
     log(f"persistent: {s_pers * 1e3:.1f} ms/step, kernel {tk_pers * 1e3:.1f} ms")
     s_plain, tk_plain = timed("plain", args.steps, 0)
     log(f"plain: {s_plain * 1e3:.1f} ms/step, kernel {tk_plain * 1e3:.1f} ms")
+
+    host_rate, host_tt, host_tk = host_api_rate(lib, d, perm)
+    log(f"host-array entry point: tTotal {host_tt * 1e3:.1f} ms, tKernel {host_tk * 1e3:.1f} ms")
 
     flops = mech.algorithmic_flops(istat)
     achieved = flops / tk_pers / 1e12
@@ -160,6 +210,8 @@ def main():
         "plain_cells_per_s": round(n / s_plain, 1),
         "speedup_vs_plain": round(s_plain / s_pers, 3),
         "s_per_chem_step": round(s_pers, 4),
+        "host_api_cells_per_s": round(host_rate, 1),
+        "host_api_t_total_s": round(host_tt, 4),
         "adj_over_fwd": None,
         "ierr_ok_frac": float((ierr == 1).mean()),
         "mean_steps_per_cell": float(istat[:, 2].mean()),

@@ -21,7 +21,6 @@ from __future__ import annotations
 
 import json
 import os
-import shutil
 import subprocess
 import sys
 import time
@@ -116,17 +115,20 @@ def plan_mechanism(mech: str, force: bool = False):
         todo.append(([HIPCC, *HIP_FLAGS, f"-I{CSRC}", "-c", str(CSRC / "gc_api.hip"),
                       "-o", str(obj)], f"{mech}/api.o"))
 
-    def link() -> Path:
-        out = lib_path(mech)
+    def link_one(out: Path):
         if force or _stale(out, objs):
             tmp = out.with_suffix(".so.tmp")
-            subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-                            *map(str, objs)], check=True)
+            subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC",
+                            f"-Wl,-soname,{out.name}", "-o", str(tmp), *map(str, objs)],
+                           check=True)
             os.replace(tmp, out)
+
+    def link() -> Path:
+        out = lib_path(mech)
+        link_one(out)
         if mech == "fullchem":
-            drop_in = LIB / "libgcchemgpu.so"
-            if force or _stale(drop_in, [out]):
-                shutil.copyfile(out, drop_in)
+            # the drop-in name the model links (-lgcchemgpu), with its own soname
+            link_one(LIB / "libgcchemgpu.so")
         return out
 
     return todo, link

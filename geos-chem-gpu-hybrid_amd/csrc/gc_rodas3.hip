@@ -651,7 +651,7 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* qu
 // ---------------------------------------------------------------------------
 // host launch wrappers of this variant
 // ---------------------------------------------------------------------------
-extern "C" hipError_t GC_SFX(gc_launch_plain)(const GcBatchArgs* a, double* ws, int block,
+extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_plain)(const GcBatchArgs* a, double* ws, int block,
                                                hipStream_t st) {
   if (a->nCell <= 0) return hipSuccess;
   const unsigned grid = ((unsigned)a->nCell + (unsigned)block - 1u) / (unsigned)block;
@@ -660,7 +660,7 @@ extern "C" hipError_t GC_SFX(gc_launch_plain)(const GcBatchArgs* a, double* ws, 
   return hipGetLastError();
 }
 
-extern "C" hipError_t GC_SFX(gc_launch_persistent)(const GcBatchArgs* a, double* ws, unsigned S,
+extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_persistent)(const GcBatchArgs* a, double* ws, unsigned S,
                                                     unsigned* queue, int block,
                                                     int cells_per_wave, int grain,
                                                     hipStream_t st) {
@@ -672,7 +672,7 @@ extern "C" hipError_t GC_SFX(gc_launch_persistent)(const GcBatchArgs* a, double*
   return hipGetLastError();
 }
 
-extern "C" void GC_SFX(gc_variant_info)(GcVariantInfo* d) {
+extern "C" GC_INTERNAL void GC_SFX(gc_variant_info)(GcVariantInfo* d) {
   d->nvar = NVAR;
   d->nfix = NFIX;
   d->nreact = NREACT;

@@ -30,14 +30,17 @@ struct GcVariantInfo {
   int waves_per_eu, seg;
 };
 
+// Internal to the library (not exported): the C-ABI in include/gcchemgpu.h is
+// the only public surface.
+#define GC_INTERNAL __attribute__((visibility("hidden")))
 #define GC_DECLARE_VARIANT(V)                                                          \
-  extern "C" hipError_t gc_launch_plain_v##V(const GcBatchArgs* a, double* ws,        \
+  extern "C" GC_INTERNAL hipError_t gc_launch_plain_v##V(const GcBatchArgs* a, double* ws,        \
                                               int block, hipStream_t st);             \
-  extern "C" hipError_t gc_launch_persistent_v##V(const GcBatchArgs* a, double* ws,   \
+  extern "C" GC_INTERNAL hipError_t gc_launch_persistent_v##V(const GcBatchArgs* a, double* ws,   \
                                                    unsigned S, unsigned* queue,       \
                                                    int block, int cells_per_wave,     \
                                                    int grain, hipStream_t st);        \
-  extern "C" void gc_variant_info_v##V(GcVariantInfo* d);
+  extern "C" GC_INTERNAL void gc_variant_info_v##V(GcVariantInfo* d);
 
 GC_DECLARE_VARIANT(1)
 GC_DECLARE_VARIANT(2)
