@@ -1,0 +1,153 @@
+"""Discrete adjoint of the batched Rodas3 solve, behind torch.autograd.Function.
+
+The forward pass is the persistent-kernel solve in its recording form: every
+accepted step's start state and step size go to a record pool in HBM (one
+record per accepted step, chained per cell).  The backward pass walks each
+cell's records from the last to the first on the GPU, recomputes the step's
+Jacobian, LU factors and stages from the recorded state, and sweeps the
+adjoint through the four stages in reverse with transposed solves on the
+same factors (KppSolveTR order, gckpp_LinearAlgebra.F90:2327-3701), the
+stoichiometry-weighted adjoint of Fun_SPLIT, the rate-constant and
+fixed-species derivatives, and the second-order term of the Jacobian's
+dependence on the state.  Step sizes and the accept/reject sequence are held
+fixed (the discrete map of the accepted steps is differentiated, as KPP's
+Rosenbrock adjoint does).
+
+Gradients are returned for the initial concentrations C (variable and fixed
+species) and RCONST.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import inputs as _inputs
+from . import lib as _lib
+from .solver import Batch, Result, _check
+
+DEFAULT_STEPS_PER_CELL = 128
+
+
+@dataclass
+class Records:
+    Y: torch.Tensor      # [cap, NVAR]
+    H: torch.Tensor      # [cap]
+    prev: torch.Tensor   # [cap] int32
+    head: torch.Tensor   # [nCell] int32
+    used: int
+    t_kernel: float
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def record_solve(b: Batch, mech: str = "fullchem", cap: int = 0) -> tuple[Result, Records]:
+    """Forward solve of the batch (persistent kernel) keeping every accepted
+    step.  The pool is sized `cap` records (default 128 per cell) and grown
+    to the exact need if that was too small."""
+    lib = _lib.load(mech)
+    nvar = _lib.mech_info(lib)[0]
+    _check(b, _lib.mech_info(lib))
+    n, dev = b.n, b.C.device
+    cap = int(cap or max(1, n * DEFAULT_STEPS_PER_CELL))
+    for _ in range(2):
+        out = Result(b.C.clone(), torch.empty((n, 8), dtype=torch.int32, device=dev),
+                     torch.empty((n, 3), dtype=torch.float64, device=dev),
+                     torch.empty((n,), dtype=torch.int32, device=dev), 0.0)
+        recY = torch.empty((cap, nvar), dtype=torch.float64, device=dev)
+        recH = torch.empty((cap,), dtype=torch.float64, device=dev)
+        prev = torch.empty((cap,), dtype=torch.int32, device=dev)
+        head = torch.empty((n,), dtype=torch.int32, device=dev)
+        used = ctypes.c_long(0)
+        tk = ctypes.c_double(0.0)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        rc = lib.gc_amd_record_batch_device(
+            n, float(b.Tstart), float(b.Tend), _p(out.C), _p(b.RCONST), _p(b.ATOL), _p(b.RTOL),
+            _p(b.ICNTRL), _p(b.RCNTRL), _p(b.Perm), _p(out.ISTAT), _p(out.RSTAT), _p(out.IERR),
+            _p(recY), _p(recH), _p(prev), _p(head), cap, ctypes.byref(used),
+            ctypes.c_void_p(stream), ctypes.byref(tk))
+        if rc == -4:
+            del recY, recH, prev
+            cap = int(used.value)
+            continue
+        if rc != 0:
+            raise RuntimeError(f"gc_amd_record_batch_device returned {rc}: {_lib.last_error(lib)}")
+        out.t_kernel = tk.value
+        return out, Records(recY, recH, prev, head, int(used.value), tk.value)
+    raise RuntimeError("record pool sizing failed")
+
+
+def adjoint_sweep(b: Batch, rec: Records, lamC: torch.Tensor, mech: str = "fullchem",
+                  perm: torch.Tensor | None = None):
+    """dL/dC0 [n, NSPEC], dL/dRCONST [n, NREACT], IERR [n] and the kernel
+    time, for dL/dC_final = lamC [n, NSPEC]."""
+    lib = _lib.load(mech)
+    n, dev = b.n, b.C.device
+    lamC = lamC.to(device=dev, dtype=torch.float64).contiguous()
+    if tuple(lamC.shape) != tuple(b.C.shape):
+        raise ValueError(f"lamC must have shape {tuple(b.C.shape)}")
+    gC = torch.empty_like(b.C)
+    gR = torch.empty_like(b.RCONST)
+    ierr = torch.empty((n,), dtype=torch.int32, device=dev)
+    perm = b.Perm if perm is None else perm.to(device=dev, dtype=torch.int32).contiguous()
+    tk = ctypes.c_double(0.0)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    rc = lib.gc_amd_adjoint_batch_device(
+        n, float(b.Tstart), float(b.Tend), _p(b.C), _p(b.RCONST), _p(b.ICNTRL), _p(perm),
+        _p(rec.Y), _p(rec.H), _p(rec.prev), _p(rec.head), int(rec.Y.shape[0]), _p(lamC),
+        _p(gC), _p(gR), _p(ierr),
+        ctypes.c_void_p(stream), ctypes.byref(tk))
+    if rc != 0:
+        raise RuntimeError(f"gc_amd_adjoint_batch_device returned {rc}: {_lib.last_error(lib)}")
+    return gC, gR, ierr, tk.value
+
+
+def records_of_cell(rec: Records, cell: int) -> np.ndarray:
+    """Step sizes of one cell's accepted steps, first to last (host copy)."""
+    head = int(rec.head[cell].item())
+    prev = rec.prev.cpu().numpy()
+    H = rec.H.cpu().numpy()
+    out = []
+    s = head
+    while s >= 0:
+        out.append(H[s])
+        s = int(prev[s])
+    return np.array(out[::-1])
+
+
+class Rodas3Solve(torch.autograd.Function):
+    """C_final = Rodas3(C0, RCONST) over [Tstart, Tend] for every cell of the
+    batch; differentiable in C0 and RCONST through the discrete adjoint."""
+
+    @staticmethod
+    def forward(ctx, C, RCONST, template: Batch, mech: str):
+        b = Batch(C.detach().contiguous(), RCONST.detach().contiguous(), template.ATOL,
+                  template.RTOL, template.ICNTRL, template.RCNTRL, template.Perm,
+                  Tstart=template.Tstart, Tend=template.Tend)
+        out, rec = record_solve(b, mech)
+        if not bool((out.IERR == 1).all()):
+            raise RuntimeError("Rodas3Solve: some cells failed in the forward solve")
+        ctx.batch, ctx.rec, ctx.mech = b, rec, mech
+        # the reverse sweep runs costly cells first
+        nacc = out.ISTAT[:, 3].cpu().numpy()
+        ctx.perm = torch.as_tensor(_inputs.cost_order(nacc), device=C.device)
+        ctx.istat = out.ISTAT
+        return out.C
+
+    @staticmethod
+    def backward(ctx, gout):
+        gC, gR, ierr, _ = adjoint_sweep(ctx.batch, ctx.rec, gout, ctx.mech, perm=ctx.perm)
+        if not bool((ierr == 1).all()):
+            raise RuntimeError("Rodas3Solve: adjoint sweep failed for some cells")
+        return gC, gR, None, None
+
+
+def solve(C: torch.Tensor, RCONST: torch.Tensor, template: Batch,
+          mech: str = "fullchem") -> torch.Tensor:
+    """Differentiable batch solve: C [n, NSPEC], RCONST [n, NREACT] (device
+    tensors, may require grad); the other inputs come from `template`."""
+    return Rodas3Solve.apply(C, RCONST, template, mech)

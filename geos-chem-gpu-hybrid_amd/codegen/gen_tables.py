@@ -52,11 +52,16 @@ Tables (int32 words):
   vector): solve rows, header {n, x_i, U(i,i) or 0, 0} then n/2
   {LU, x_j, LU, x_j} with n a multiple of 4, absolute offsets.
   (gckpp_LinearAlgebra.F90:645-2311, 2327-3701)
+* discrete adjoint: ``adj_nu`` (nu = S^T mu per reaction, S the net
+  stoichiometry of the Fun_SPLIT sums, cf. CalcStoichNum,
+  gckpp_Function.F90:7271-12790) and ``adj_mono_y``/``adj_mono_yn`` (the
+  rate monomials with their factor, direction and gradient offsets).
 """
 from __future__ import annotations
 
 import json
 import struct
+from fractions import Fraction
 import sys
 from pathlib import Path
 
@@ -82,6 +87,16 @@ def layout(ir) -> dict:
     E["ZERO"] = E["LU"] + nnz  # holds 0.0 (padding operand)
     E["SINK"] = E["ZERO"] + 1  # written by padding stores, never read
     E["TOTAL"] = E["SINK"] + 1
+    # discrete adjoint: the forward block followed by the adjoint vectors
+    E["LAM"] = E["TOTAL"]          # lambda = dL/dY at the end of the step
+    E["LK"] = E["LAM"] + nvar      # 4 stage adjoints lambda_K
+    E["GY"] = E["LK"] + 4 * nvar   # dL/dY accumulated over the step
+    E["LP"] = E["GY"] + nvar       # J(P)^T mu of one stage / second-order part
+    E["DIR"] = E["LP"] + nvar      # stage vector K_s as a direction
+    E["NU"] = E["DIR"] + nvar      # S^T mu per reaction
+    E["GK"] = E["NU"] + nreact     # dL/dRCONST
+    E["GF"] = E["GK"] + nreact     # dL/dFIX
+    E["ADJ_TOTAL"] = E["GF"] + nfix
     return E
 
 
@@ -240,6 +255,118 @@ class Tables:
         ops += [0, 0, 0, 0]
         return ops
 
+    # -- discrete adjoint ------------------------------------------------------
+    def rate_monomial(self, r):
+        """(literal or None, has RCT, [(kind, idx)] data factors) of A(r)."""
+        lit, has_k, data = None, False, []
+        for kind, idx in self.ir["rate"][r]:
+            if kind == "lit":
+                lit = self.lit_value(idx)
+            elif kind == "RCT":
+                if idx != r:
+                    raise ValueError(f"A({r + 1}) reads RCT({idx + 1})")
+                has_k = True
+            else:
+                data.append((kind, idx))
+        if not 1 <= len(data) <= 3:
+            raise ValueError(f"A({r + 1}) has {len(data)} data factors")
+        return lit, has_k, data
+
+    def stoichiometry(self):
+        """Net stoichiometric coefficients S[i][r] (dict per species) from the
+        Fun_SPLIT sums: a P_VAR term c*A(r) adds c; a D_VAR term of species i
+        is m*A(r)/V(i), found by its monomial, and subtracts m.  The decimal
+        literals are summed exactly and rounded once, as a literal net
+        coefficient of CalcStoichNum would be."""
+        ir = self.ir
+        exact = lambda text: Fraction(str(text).replace("d", "e").replace("D", "e"))
+        mono = {}
+        for r in range(ir["nreact"]):
+            lit, has_k, data = self.rate_monomial(r)
+            key = (has_k and r, tuple(sorted(data)))
+            mono.setdefault(key, []).append((r, lit))
+        S = [dict() for _ in range(ir["nvar"])]
+        for i, terms in enumerate(ir["prod"]):
+            for sign, prod in terms:
+                c = Fraction(1)
+                (kind, r), = [f for f in prod if f[0] == "A"]
+                for f in prod:
+                    if f[0] == "lit":
+                        c = exact(f[1])
+                S[i][r] = S[i].get(r, 0) + sign * c
+        for i, terms in enumerate(ir["dest"]):
+            for sign, prod in terms:
+                lit, k, data = None, None, [("V", i)]
+                for kind, idx in prod:
+                    if kind == "lit":
+                        lit = exact(idx)
+                    elif kind == "RCT":
+                        k = idx
+                    else:
+                        data.append((kind, idx))
+                key = (k if k is not None else False, tuple(sorted(data)))
+                cands = mono.get(key, [])
+                if k is None:
+                    cands = [(r, rl) for r, rl in cands if rl is not None]
+                if len(cands) != 1:
+                    raise ValueError(f"D_VAR({i + 1}) term {prod}: {len(cands)} matching rates")
+                r, rl = cands[0]
+                m = lit if lit is not None else Fraction(1)
+                if k is None:  # literal rate: its literal is part of the D term
+                    m = m / exact(self._rate_lit_text(r))
+                S[i][r] = S[i].get(r, 0) - sign * m
+        return [{r: float(c) for r, c in row.items()} for row in S]
+
+    def _rate_lit_text(self, r):
+        (text,) = [v for kind, v in self.ir["rate"][r] if kind == "lit"]
+        return text
+
+    def adj_nu(self):
+        """Per reaction {n, NU(r), 0, 0} + n terms {TMP(i), 0, S(i,r)}: the
+        reaction weights nu = S^T mu of a vector mu held in TMP."""
+        cols = [[] for _ in range(self.ir["nreact"])]
+        for i, row in enumerate(self.stoichiometry()):
+            for r, c in sorted(row.items()):
+                if c != 0.0:
+                    cols[r].append((i, c))
+        out = []
+        for r, col in enumerate(cols):
+            out += [len(col), self.off("NU", r), 0, 0]
+            for i, c in col:
+                out += [self.off("TMP", i), 0, *_coef_words(c)]
+        return out
+
+    def adj_mono(self, vbase):
+        """Per reaction a 16-word record {nf, has_k, lit, NU(r), RCT(r), GK(r),
+        x1..x3, d1..d3, g1..g3, 0}: the monomial's data factors at the point
+        vbase (Y or YNEW; FIX for fixed species), their direction components
+        (DIR for variables, ZERO for fixed species) and their gradient targets
+        (LP for variables, GF for fixed species).  Unused slots read ZERO and
+        write SINK."""
+        z, sink = self.off("ZERO", 0), self.off("SINK", 0)
+        out = []
+        for r in range(self.ir["nreact"]):
+            lit, has_k, data = self.rate_monomial(r)
+            x, d, g = [], [], []
+            for kind, idx in data:
+                if kind == "V":
+                    x.append(self.off(vbase, idx))
+                    d.append(self.off("DIR", idx))
+                    g.append(self.off("LP", idx))
+                else:
+                    x.append(self.off("FIX", idx))
+                    d.append(z)
+                    g.append(self.off("GF", idx))
+            while len(x) < 3:
+                x.append(z)
+                d.append(z)
+                g.append(sink)
+            out += [len(data), 1 if has_k else 0,
+                    self.lit_idx(lit) if lit is not None else -1, self.off("NU", r),
+                    self.off("RCT", r) if has_k else z, self.off("GK", r) if has_k else sink,
+                    *x, *d, *g, 0]
+        return out
+
     # -- solves ----------------------------------------------------------------
     # One table per target vector, absolute offsets: padding pairs read ZERO
     # for both operands, so x - 0.0*0.0 = x exactly.
@@ -334,6 +461,8 @@ def generate(ir: dict) -> str:
         sizes[f"fwd{stage}"] = len(fw)
     tr1, n1, tr2, n2 = t.tr("TMP")
     body += [_arr("gc_tr1", tr1), _arr("gc_tr2", tr2)]
+    body += [_arr("gc_adj_nu", t.adj_nu()), _arr("gc_adj_mono_y", t.adj_mono("Y")),
+             _arr("gc_adj_mono_yn", t.adj_mono("YNEW"))]
     for s in SEGMENTS:
         ops = t.lu(s)
         body.append(_arr(f"gc_lu{s}", ops))

@@ -56,7 +56,7 @@ struct Tuning {
   int blocks_per_cu = 0;
   int cells_per_wave = 64;
   int grain = 64;
-  int variant = 2;
+  int variant = 1;
 };
 std::mutex g_tune_mu;
 Tuning g_tune;
@@ -150,6 +150,11 @@ __global__ void k_check_perm(const int* perm, int n, int* bad) {
   }
 }
 
+__global__ void k_fill_i32(int* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 __global__ void k_zero_u32(unsigned* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -199,6 +204,27 @@ unsigned slots_for(DevCtx& c, int kernel, const Tuning& t, int n) {
   return grid * (unsigned)t.block;
 }
 
+// Launch order must be a map into 1..nCell (a bad entry would address outside
+// the caller's arrays).  Returns 0, -1 (bad entry), -2 or -3.
+int check_perm(DevCtx& c, const int* Perm, int nCell, hipStream_t st) {
+  hipError_t e;
+  if ((e = c.flag.ensure(sizeof(int))) != hipSuccess) { set_error("hipMalloc(flag)", e); return -2; }
+  if ((e = hipMemsetAsync(c.flag.p, 0, sizeof(int), st)) != hipSuccess) { set_error("hipMemsetAsync", e); return -3; }
+  hipLaunchKernelGGL(k_check_perm, dim3((nCell + 255) / 256), dim3(256), 0, st, Perm, nCell,
+                     (int*)c.flag.p);
+  int bad = 0;
+  if ((e = hipMemcpyAsync(&bad, c.flag.p, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess) {
+    set_error("perm check", e);
+    return -3;
+  }
+  if (bad) {
+    set_error("Perm entry outside 1..nCell");
+    return -1;
+  }
+  return 0;
+}
+
 int device_solve(DevCtx& c, int kernel, int chunk, const GcBatchArgs& b, hipStream_t st,
                  double* tKernel) {
   if (tKernel) *tKernel = 0.0;
@@ -211,21 +237,9 @@ int device_solve(DevCtx& c, int kernel, int chunk, const GcBatchArgs& b, hipStre
   const Tuning t = current_tuning();
   hipError_t e;
 
-  // launch order must be a map into 1..nCell (a bad entry would address
-  // outside the caller's arrays)
-  if ((e = c.flag.ensure(sizeof(int))) != hipSuccess) { set_error("hipMalloc(flag)", e); return -2; }
-  if ((e = hipMemsetAsync(c.flag.p, 0, sizeof(int), st)) != hipSuccess) { set_error("hipMemsetAsync", e); return -3; }
-  hipLaunchKernelGGL(k_check_perm, dim3((b.nCell + 255) / 256), dim3(256), 0, st, b.Perm, b.nCell,
-                     (int*)c.flag.p);
-  int bad = 0;
-  if ((e = hipMemcpyAsync(&bad, c.flag.p, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess ||
-      (e = hipStreamSynchronize(st)) != hipSuccess) {
-    set_error("perm check", e);
-    return -3;
-  }
-  if (bad) {
-    set_error("gc_amd_integrate_batch_device: Perm entry outside 1..nCell");
-    return -1;
+  {
+    const int rc = check_perm(c, b.Perm, b.nCell, st);
+    if (rc) return rc;
   }
 
   const int span = (chunk > 0 && chunk < b.nCell) ? chunk : b.nCell;
@@ -302,9 +316,145 @@ void fill_ierr(int* IERR, int n, int code) {
   for (int i = 0; i < n; ++i) IERR[i] = code;
 }
 
+// Recording forward solve (persistent kernel, one launch, no chunking).
+int record_solve(DevCtx& c, const GcBatchArgs& b, const GcRecord& rec_in, long* used,
+                 hipStream_t st, double* tKernel) {
+  if (tKernel) *tKernel = 0.0;
+  if (used) *used = 0;
+  if (b.nCell == 0) return 0;
+  const Tuning t = current_tuning();
+  int rc = check_perm(c, b.Perm, b.nCell, st);
+  if (rc) return rc;
+  hipError_t e;
+  const unsigned S = slots_for(c, GC_AMD_KERNEL_PERSISTENT, t, b.nCell);
+  if ((e = c.ws.ensure((size_t)S * (size_t)ws_per_slot() * sizeof(double))) != hipSuccess) {
+    set_error("hipMalloc(workspace)", e);
+    return -2;
+  }
+  if ((e = c.queue.ensure(256)) != hipSuccess) { set_error("hipMalloc(queue)", e); return -2; }
+  unsigned* q = (unsigned*)c.queue.p;
+  GcRecord rec = rec_in;
+  rec.count = q + 1;  // the second queue word counts records
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, q, 2);
+  hipLaunchKernelGGL(k_fill_i32, dim3((b.nCell + 255) / 256), dim3(256), 0, st, rec.head,
+                     b.nCell, -1);
+  if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+  const unsigned waves = S / 64u;
+  unsigned fair = ((unsigned)b.nCell + waves - 1u) / waves;
+  if (fair < 1u) fair = 1u;
+  const int grain = (unsigned)t.grain > fair ? (int)fair : t.grain;
+  e = (t.variant == 2)
+          ? gc_launch_record_v2(&b, (double*)c.ws.p, S, q, t.block, t.cells_per_wave, grain, &rec, st)
+          : gc_launch_record_v1(&b, (double*)c.ws.p, S, q, t.block, t.cells_per_wave, grain, &rec, st);
+  if (e != hipSuccess) { set_error("launch", e); return -3; }
+  if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+  unsigned count = 0;
+  if ((e = hipMemcpyAsync(&count, rec.count, sizeof(unsigned), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess) {
+    set_error("record solve", e);
+    return -3;
+  }
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess && tKernel) *tKernel = ms * 1e-3;
+  if (used) *used = (long)count;
+  if (count > rec.cap) {
+    set_error("record pool too small: retry with cap >= *used");
+    return -4;
+  }
+  return 0;
+}
+
+int adjoint_solve(DevCtx& c, const GcAdjArgs& a, hipStream_t st, double* tKernel) {
+  if (tKernel) *tKernel = 0.0;
+  if (a.nCell == 0) return 0;
+  const Tuning t = current_tuning();
+  int rc = check_perm(c, a.Perm, a.nCell, st);
+  if (rc) return rc;
+  hipError_t e;
+  const size_t slots = ((size_t)a.nCell + 63u) & ~(size_t)63u;
+  const long per = variant_info(t.variant).adj_doubles_per_slot;
+  if ((e = c.ws.ensure(slots * (size_t)per * sizeof(double))) != hipSuccess) {
+    set_error("hipMalloc(workspace)", e);
+    return -2;
+  }
+  if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+  e = (t.variant == 2) ? gc_launch_adjoint_v2(&a, (double*)c.ws.p, 128, st)
+                       : gc_launch_adjoint_v1(&a, (double*)c.ws.p, 128, st);
+  if (e != hipSuccess) { set_error("launch", e); return -3; }
+  if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) { set_error("adjoint", e); return -3; }
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess && tKernel) *tKernel = ms * 1e-3;
+  return 0;
+}
+
+// Device context of the calling thread, initialised, locked by the caller;
+// orders the library stream after the caller's stream.
+DevCtx* enter(void* stream, int* rc) {
+  int d = -1;
+  DevCtx* cp = current_ctx(&d);
+  if (!cp) {
+    set_error("no current device");
+    *rc = -3;
+    return nullptr;
+  }
+  cp->mu.lock();
+  hipError_t e = ctx_init(*cp, d);
+  if (e == hipSuccess) e = hipEventRecord(cp->ev_in, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(cp->stream, cp->ev_in, 0);
+  if (e != hipSuccess) {
+    set_error("device init / stream ordering", e);
+    cp->mu.unlock();
+    *rc = -3;
+    return nullptr;
+  }
+  *rc = 0;
+  return cp;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double* C,
+                               const double* RCONST, const double* ATOL, const double* RTOL,
+                               const int* ICNTRL, const double* RCNTRL, const int* Perm,
+                               int* ISTAT, double* RSTAT, int* IERR, double* recY,
+                               double* recH, int* recPrev, int* head, long cap, long* used,
+                               void* stream, double* tKernel) {
+  if (nCell < 0 || cap < 0 || cap > 0x7fffffffL) {
+    set_error("gc_amd_record_batch_device: bad argument");
+    return -1;
+  }
+  int rc = 0;
+  DevCtx* cp = enter(stream, &rc);
+  if (!cp) return rc;
+  GcBatchArgs b{nCell, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR};
+  GcRecord rec{recY, recH, recPrev, head, nullptr, (unsigned)cap};
+  rc = record_solve(*cp, b, rec, used, cp->stream, tKernel);
+  cp->mu.unlock();
+  return rc;
+}
+
+int gc_amd_adjoint_batch_device(int nCell, double Tstart, double Tend, const double* C,
+                                const double* RCONST, const int* ICNTRL, const int* Perm,
+                                const double* recY, const double* recH, const int* recPrev,
+                                const int* head, long cap, const double* lamC, double* gC,
+                                double* gRCONST, int* IERR, void* stream, double* tKernel) {
+  if (nCell < 0 || cap < 0 || cap > 0x7fffffffL) {
+    set_error("gc_amd_adjoint_batch_device: bad argument");
+    return -1;
+  }
+  int rc = 0;
+  DevCtx* cp = enter(stream, &rc);
+  if (!cp) return rc;
+  GcAdjArgs a{nCell, Tstart, Tend, C, RCONST, ICNTRL, Perm, recY, recH, recPrev, head,
+              (unsigned)cap, lamC, gC, gRCONST, IERR};
+  rc = adjoint_solve(*cp, a, cp->stream, tKernel);
+  cp->mu.unlock();
+  return rc;
+}
+
 
 int gc_gpu_set_device(int dev) {
   int n = 0;

@@ -113,14 +113,15 @@ struct Blk {
   __device__ __forceinline__ void lds_st(int off, double v) const { *(double*)((char*)lds + off) = v; }
 };
 
-__device__ __forceinline__ Blk make_blk(double* ws, unsigned wave, int lane, double* lds_wave) {
+__device__ __forceinline__ Blk make_blk(double* ws, unsigned wave, int lane, double* lds_wave,
+                                        int total = E_TOTAL) {
   const unsigned long long base =
-      (unsigned long long)(ws + (size_t)wave * (size_t)E_TOTAL * 64ull);
+      (unsigned long long)(ws + (size_t)wave * (size_t)total * 64ull);
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)base);
   const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(base >> 32));
   void* p = (void*)(((unsigned long long)hi << 32) | lo);
   Blk b;
-  b.r = __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)(E_TOTAL * 512), 0x00020000);
+  b.r = __builtin_amdgcn_make_buffer_rsrc(p, (short)0, total * 512, 0x00020000);
   b.voff = lane * 8;
   b.lds = lds_wave + lane;
   return b;
@@ -362,21 +363,25 @@ __device__ void step_functions(const Blk& b, const Opts& o, double T, int* ist) 
   }
 }
 
-// The four stages after a successful factorisation, then Ynew, Yerr and the
-// error norm (gckpp_BatchIntegrator.F90:594-665).  Returns Err.
-__device__ double stages_and_error(const Blk& b, const Opts& o, double Dir, double H,
-                                   const double* atol, const double* rtol, int* ist) {
+// Ynew = Y + sum_j A_sj K_j, the point of stage s >= 2 (ros_NewF).
+__device__ void stage_point(const Blk& b, int s) {
+  for (int k = 0; k < NVAR; ++k) {
+    double y = b.ld(O_Y + k * 512);
+    for (int j = 0; j < s; ++j) y = y + ros_A[s * (s - 1) / 2 + j] * b.ld(O_K + j * O_VEC + k * 512);
+    b.st(O_YNEW + k * 512, y);
+  }
+}
+
+// The four stages after a successful factorisation
+// (gckpp_BatchIntegrator.F90:594-643).
+__device__ void stages(const Blk& b, bool autonomous, double Dir, double H, int* ist) {
   for (int s = 0; s < 4; ++s) {
     const int Ks = O_K + s * O_VEC;
     if (s == 0 || s == 1) {
       // stage 1: Fcn = Fcn0; stage 2 has ros_NewF = .FALSE. so Fcn is still Fcn0
       for (int k = 0; k < NVAR; ++k) b.st(Ks + k * 512, b.ld(O_FCN0 + k * 512));
     } else {
-      for (int k = 0; k < NVAR; ++k) {
-        double y = b.ld(O_Y + k * 512);
-        for (int j = 0; j < s; ++j) y = y + ros_A[s * (s - 1) / 2 + j] * b.ld(O_K + j * O_VEC + k * 512);
-        b.st(O_YNEW + k * 512, y);
-      }
+      stage_point(b, s);
       fun(b, true, Ks);
       ist[0] += 1;
     }
@@ -385,7 +390,7 @@ __device__ double stages_and_error(const Blk& b, const Opts& o, double Dir, doub
       for (int k = 0; k < NVAR; ++k)
         b.st(Ks + k * 512, b.ld(Ks + k * 512) + HC * b.ld(O_K + j * O_VEC + k * 512));
     }
-    if (!o.autonomous && ros_Gamma[s] != 0.0) {
+    if (!autonomous && ros_Gamma[s] != 0.0) {
       const double HG = (Dir * H) * ros_Gamma[s];
       for (int k = 0; k < NVAR; ++k)
         b.st(Ks + k * 512, b.ld(Ks + k * 512) + HG * b.ld(O_DFDT + k * 512));
@@ -393,6 +398,13 @@ __device__ double stages_and_error(const Blk& b, const Opts& o, double Dir, doub
     solve(b, s);
     ist[6] += 1;
   }
+}
+
+// The stages, then Ynew, Yerr and the error norm
+// (gckpp_BatchIntegrator.F90:594-665).  Returns Err.
+__device__ double stages_and_error(const Blk& b, const Opts& o, double Dir, double H,
+                                   const double* atol, const double* rtol, int* ist) {
+  stages(b, o.autonomous, Dir, H, ist);
   double Err = 0.0;
   for (int k = 0; k < NVAR; ++k) {
     const double y = b.ld(O_Y + k * 512);
@@ -502,10 +514,33 @@ GC_SFX(gc_rodas3_plain)(GcBatchArgs a, double* ws) {
   scatter(a, cell, b, ist, rst, ierr);
 }
 
-extern "C" __global__ void
-__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
-GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* queue,
-                             int cells_per_wave, int grain) {
+namespace {
+
+// Record of an accepted step (recording form): the state at its start and its
+// H go to a slot taken from the record pool (one atomic per wave), chained to
+// the cell's previous record.
+__device__ __forceinline__ void record_step(const Blk& b, const GcRecord& rec, bool acc,
+                                            double H, int lane, int& head) {
+  const unsigned long long mask = __ballot(acc);
+  if (mask == 0ull) return;
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(rec.count, (unsigned)__popcll(mask));
+  base = __shfl(base, leader);
+  if (!acc) return;
+  const unsigned slot = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+  if (slot >= rec.cap) return;  // pool exhausted: the host sees count > cap
+  double* y = rec.Y + (size_t)slot * NVAR;
+  for (int k = 0; k < NVAR; ++k) y[k] = b.ld(O_Y + k * 512);
+  rec.H[slot] = H;
+  rec.prev[slot] = head;
+  head = (int)slot;
+}
+
+template <bool REC>
+__device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws, unsigned S,
+                                                unsigned* queue, int cells_per_wave, int grain,
+                                                const GcRecord& rec) {
   extern __shared__ double lds[];
   const unsigned slot = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -522,6 +557,7 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* qu
   bool rejLast = false, rejMore = false;
   bool exhausted = !lane_on;
   unsigned rsv = 0, rsv_end = 0;  // wave-uniform reservation [rsv, rsv_end)
+  int head = -1;                  // last record of the lane's cell (recording form)
 
   for (;;) {
     // ---- refill free lanes from the queue (cost order) -------------------
@@ -554,6 +590,7 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* qu
             scatter(a, cell, b, ist, rst, e);
             cell = -1;  // stays free: takes another cell
           } else {
+            head = -1;
             Nsteps = 0;
             T = a.Tstart;
             H = dmin(dmax(fabs(o.Hmin), fabs(o.Hstart)), fabs(o.Hmax));
@@ -581,6 +618,7 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* qu
       else if (((T + 0.1 * H) == T) || (H <= ROUNDOFF)) ierr_now = -7;
       if (ierr_now != 0) {
         scatter(a, cell, b, ist, rst, ierr_now);
+        if (REC) rec.head[cell] = head;
         phase = PH_FREE;
         cell = -1;
       } else {
@@ -610,6 +648,7 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* qu
           phase = PH_SING;
         } else {
           scatter(a, cell, b, ist, rst, -8);
+          if (REC) rec.head[cell] = head;
           phase = PH_FREE;
           cell = -1;
         }
@@ -617,13 +656,19 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* qu
         factored = true;
       }
     }
+    bool accepted = false;
+    double Hnew = 0.0;
     if (factored) {
       const double Err = stages_and_error(b, o, Dir, H, a.ATOL, a.RTOL, ist);
       const double Fac = dmin(o.FacMax, dmax(o.FacMin, o.FacSafe / gc_pow_third(Err)));
-      double Hnew = H * Fac;
+      Hnew = H * Fac;
       Nsteps += 1;
       ist[2] += 1;
-      if ((Err <= 1.0) || (H <= o.Hmin)) {
+      accepted = (Err <= 1.0) || (H <= o.Hmin);
+    }
+    if (REC) record_step(b, rec, accepted, H, lane, head);
+    if (factored) {
+      if (accepted) {
         ist[3] += 1;
         accept_y(b, o);
         T = T + Dir * H;
@@ -646,6 +691,222 @@ GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* qu
       }
     }
   }
+}
+
+}  // namespace
+
+extern "C" __global__ void
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* queue,
+                             int cells_per_wave, int grain) {
+  persistent_body<false>(a, ws, S, queue, cells_per_wave, grain, GcRecord{});
+}
+
+// Recording form: also keeps every accepted step's start state and H for the
+// discrete adjoint.
+extern "C" __global__ void
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_record)(GcBatchArgs a, double* ws, unsigned S, unsigned* queue,
+                         int cells_per_wave, int grain, GcRecord rec) {
+  persistent_body<true>(a, ws, S, queue, cells_per_wave, grain, rec);
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the accepted Rodas3 steps (step sizes held fixed)
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int O_TMP = MECH_E_TMP * 512;
+constexpr int O_LAM = MECH_E_LAM * 512;
+constexpr int O_LK = MECH_E_LK * 512;
+constexpr int O_GY = MECH_E_GY * 512;
+constexpr int O_LP = MECH_E_LP * 512;
+constexpr int O_DIR = MECH_E_DIR * 512;
+constexpr int O_GK = MECH_E_GK * 512;
+constexpr int O_GF = MECH_E_GF * 512;
+
+// KppSolveTR (gckpp_LinearAlgebra.F90:2327-3701) in place on TMP: U^T, then L^T.
+__device__ void solve_tr(const Blk& b) {
+  solve_rows<true>(b, tab(gc_tr1), MECH_NTR1);
+  solve_rows<false>(b, tab(gc_tr2), MECH_NTR2);
+}
+
+// nu(r) = sum_i S(i,r) mu(i) for mu in TMP, S the net stoichiometry.
+__device__ void adj_nu(const Blk& b) {
+  ctab_t* t = tab(gc_adj_nu);
+  for (int r = 0; r < NREACT; ++r) {
+    const int n = t[0], dst = t[1];
+    t += 4;
+    b.st(dst, n > 0 ? sum_terms(b, t, n) : 0.0);
+  }
+}
+
+__device__ __forceinline__ void acc(const Blk& b, int g, double v) { b.st(g, b.ld(g) + v); }
+
+// Rate monomials A_r = c k x1 [x2 [x3]] with weights nu_r:
+//  first order  (SECOND false): g_f += nu c k prod_{h != f} x_h, GK_r += nu c prod x
+//  second order (SECOND true) : the gradient of nu_r (grad A_r . d) for the
+//               direction d in DIR: g_f += nu c k d/dx_f (sum_g prod_{h != g} x_h d_g),
+//               GK_r += nu c sum_g prod_{h != g} x_h d_g
+// Targets g are LP (variables) or GF (fixed species); see codegen adj_mono.
+template <bool SECOND>
+__device__ void adj_mono(const Blk& b, const int* table) {
+  ctab_t* q = tab(table);
+  for (int r = 0; r < NREACT; ++r, q += 16) {
+    const int nf = q[0], hk = q[1], li = q[2];
+    const double nu = b.ld(q[3]);
+    const double c = li >= 0 ? ((clit_t*)gc_lits)[li] : 1.0;
+    const double k = hk ? b.ld(q[4]) : 1.0;
+    const double x1 = b.ld(q[6]);
+    const double w = nu * c * k;
+    if (!SECOND) {
+      if (nf == 1) {
+        acc(b, q[12], w);
+        if (hk) acc(b, q[5], nu * c * x1);
+      } else if (nf == 2) {
+        const double x2 = b.ld(q[7]);
+        acc(b, q[12], w * x2);
+        acc(b, q[13], w * x1);
+        if (hk) acc(b, q[5], nu * c * (x1 * x2));
+      } else {
+        const double x2 = b.ld(q[7]), x3 = b.ld(q[8]);
+        acc(b, q[12], w * (x2 * x3));
+        acc(b, q[13], w * (x1 * x3));
+        acc(b, q[14], w * (x1 * x2));
+        if (hk) acc(b, q[5], nu * c * (x1 * x2 * x3));
+      }
+    } else {
+      const double d1 = b.ld(q[9]);
+      if (nf == 1) {
+        if (hk) acc(b, q[5], nu * c * d1);
+      } else if (nf == 2) {
+        const double x2 = b.ld(q[7]), d2 = b.ld(q[10]);
+        acc(b, q[12], w * d2);
+        acc(b, q[13], w * d1);
+        if (hk) acc(b, q[5], nu * c * (x2 * d1 + x1 * d2));
+      } else {
+        const double x2 = b.ld(q[7]), x3 = b.ld(q[8]);
+        const double d2 = b.ld(q[10]), d3 = b.ld(q[11]);
+        acc(b, q[12], w * (x3 * d2 + x2 * d3));
+        acc(b, q[13], w * (x3 * d1 + x1 * d3));
+        acc(b, q[14], w * (x2 * d1 + x1 * d2));
+        if (hk) acc(b, q[5], nu * c * (x2 * x3 * d1 + x1 * x3 * d2 + x1 * x2 * d3));
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void vzero(const Blk& b, int o) {
+  for (int k = 0; k < NVAR; ++k) b.st(o + k * 512, 0.0);
+}
+
+// One accepted step in reverse.  On entry Y holds the step's start state, the
+// LU factors of M = I/(gamma*Dir*H) - J(Y) and the stage vectors K_1..K_4 have
+// been recomputed, LAM = dL/dY at the end of the step.  On exit LAM = dL/dY
+// at its start; GF and GK have collected the step's terms.
+__device__ void adj_step(const Blk& b, double Dir, double H, bool clamp) {
+  const double dh = Dir * H;
+  if (clamp) {  // Y' = max(Ynew, 0): no sensitivity where Ynew <= 0
+    for (int k = 0; k < NVAR; ++k) {
+      double yn = b.ld(O_Y + k * 512);
+      for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * b.ld(O_K + j * O_VEC + k * 512);
+      if (!(yn > 0.0)) b.st(O_LAM + k * 512, 0.0);
+    }
+  }
+  for (int k = 0; k < NVAR; ++k) {
+    const double l = b.ld(O_LAM + k * 512);
+    for (int j = 0; j < 4; ++j) b.st(O_LK + j * O_VEC + k * 512, ros_M[j] * l);
+    b.st(O_GY + k * 512, l);
+  }
+  for (int s = 3; s >= 0; --s) {
+    // mu_s = M^-T lambda_Ks, nu = S^T mu_s
+    for (int k = 0; k < NVAR; ++k) b.st(O_TMP + k * 512, b.ld(O_LK + s * O_VEC + k * 512));
+    solve_tr(b);
+    adj_nu(b);
+    // first order at the stage point: J(P_s)^T mu_s, d/dk, d/dF
+    if (s >= 2) stage_point(b, s);
+    vzero(b, O_LP);
+    adj_mono<false>(b, s >= 2 ? gc_adj_mono_yn : gc_adj_mono_y);
+    for (int k = 0; k < NVAR; ++k) {
+      const double lp = b.ld(O_LP + k * 512), mu = b.ld(O_TMP + k * 512);
+      b.st(O_GY + k * 512, b.ld(O_GY + k * 512) + lp);
+      for (int j = 0; j < s; ++j) {
+        const int o = O_LK + j * O_VEC + k * 512;
+        double v = b.ld(o) + (ros_C[s * (s - 1) / 2 + j] / dh) * mu;
+        if (s >= 2) v = v + ros_A[s * (s - 1) / 2 + j] * lp;
+        b.st(o, v);
+      }
+    }
+    // second order: d/dY of mu_s^T J(Y) K_s
+    for (int k = 0; k < NVAR; ++k) b.st(O_DIR + k * 512, b.ld(O_K + s * O_VEC + k * 512));
+    vzero(b, O_LP);
+    adj_mono<true>(b, gc_adj_mono_y);
+    for (int k = 0; k < NVAR; ++k)
+      b.st(O_GY + k * 512, b.ld(O_GY + k * 512) + b.ld(O_LP + k * 512));
+  }
+  for (int k = 0; k < NVAR; ++k) b.st(O_LAM + k * 512, b.ld(O_GY + k * 512));
+}
+
+}  // namespace
+
+// One thread per cell in Perm order: walks the cell's recorded steps from the
+// last to the first, recomputing each step's factors and stages from its
+// start state, and sweeps the adjoint through it.
+extern "C" __global__ void
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
+  extern __shared__ double lds[];
+  const unsigned ic = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const Blk b = make_blk(ws, ic >> 6, lane, lds + (threadIdx.x >> 6) * ((GC_SEG + 1) * 64),
+                         MECH_E_ADJ_TOTAL);
+  if (ic >= (unsigned)a.nCell) return;
+  const long cell = (long)a.Perm[ic] - 1;
+  const double* c = a.C + cell * NSPEC;
+  const double* rc = a.RCONST + cell * NREACT;
+  const double* lam = a.lamC + cell * NSPEC;
+  double* gc = a.gC + cell * NSPEC;
+  double* gk = a.gRCONST + cell * NREACT;
+  const int* icu = a.ICNTRL + cell * 20;
+  // option merge as decode_opts: user values where non-zero
+  const bool autonomous = icu[0] != 0;
+  const bool clamp = icu[15] == 1;
+  if (!autonomous) {  // the dFdT stage terms are not differentiated
+    for (int i = 0; i < NSPEC; ++i) gc[i] = 0.0;
+    for (int r = 0; r < NREACT; ++r) gk[r] = 0.0;
+    a.IERR[cell] = -3;
+    return;
+  }
+  for (int j = 0; j < NFIX; ++j) b.st(O_FIX + j * 512, c[NVAR + j]);
+  for (int r = 0; r < NREACT; ++r) b.st(O_RCT + r * 512, rc[r]);
+  b.st(MECH_E_ZERO * 512, 0.0);
+  for (int i = 0; i < NVAR; ++i) b.st(O_LAM + i * 512, lam[i]);
+  for (int j = 0; j < NFIX; ++j) b.st(O_GF + j * 512, lam[NVAR + j]);
+  for (int r = 0; r < NREACT; ++r) b.st(O_GK + r * 512, 0.0);
+  const double Dir = (a.Tend >= a.Tstart) ? 1.0 : -1.0;
+  int ist[8];
+  int ierr = 1;
+  unsigned walked = 0;
+  for (int slot = a.head[cell]; slot >= 0; slot = a.recPrev[slot]) {
+    if ((unsigned)slot >= a.cap || ++walked > a.cap) {  // not a chain of this pool
+      ierr = -5;
+      break;
+    }
+    const double* y = a.recY + (size_t)slot * NVAR;
+    for (int i = 0; i < NVAR; ++i) b.st(O_Y + i * 512, y[i]);
+    const double H = a.recH[slot];
+    fun(b, false, O_FCN0);
+    if (jac_lu(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
+      ierr = -8;
+      break;
+    }
+    stages(b, true, Dir, H, ist);
+    adj_step(b, Dir, H, clamp);
+  }
+  for (int i = 0; i < NVAR; ++i) gc[i] = b.ld(O_LAM + i * 512);
+  for (int j = 0; j < NFIX; ++j) gc[NVAR + j] = b.ld(O_GF + j * 512);
+  for (int r = 0; r < NREACT; ++r) gk[r] = b.ld(O_GK + r * 512);
+  a.IERR[cell] = ierr;
 }
 
 // ---------------------------------------------------------------------------
@@ -672,12 +933,34 @@ extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_persistent)(const GcBatchArgs
   return hipGetLastError();
 }
 
+extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_record)(const GcBatchArgs* a, double* ws,
+                                                         unsigned S, unsigned* queue, int block,
+                                                         int cells_per_wave, int grain,
+                                                         const GcRecord* rec, hipStream_t st) {
+  if (a->nCell <= 0) return hipSuccess;
+  const unsigned grid = S / (unsigned)block;
+  const size_t lds = (size_t)(block / 64) * (GC_SEG + 1) * 64 * sizeof(double);
+  hipLaunchKernelGGL(GC_SFX(gc_rodas3_record), dim3(grid), dim3(block), lds, st, *a, ws, S,
+                     queue, cells_per_wave, grain, *rec);
+  return hipGetLastError();
+}
+
+extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_adjoint)(const GcAdjArgs* a, double* ws,
+                                                          int block, hipStream_t st) {
+  if (a->nCell <= 0) return hipSuccess;
+  const unsigned grid = ((unsigned)a->nCell + (unsigned)block - 1u) / (unsigned)block;
+  const size_t lds = (size_t)(block / 64) * (GC_SEG + 1) * 64 * sizeof(double);
+  hipLaunchKernelGGL(GC_SFX(gc_rodas3_adjoint), dim3(grid), dim3(block), lds, st, *a, ws);
+  return hipGetLastError();
+}
+
 extern "C" GC_INTERNAL void GC_SFX(gc_variant_info)(GcVariantInfo* d) {
   d->nvar = NVAR;
   d->nfix = NFIX;
   d->nreact = NREACT;
   d->nnz = MECH_NNZ;
   d->ws_doubles_per_slot = E_TOTAL;
+  d->adj_doubles_per_slot = MECH_E_ADJ_TOTAL;
   d->waves_per_eu = GC_WPE;
   d->seg = GC_SEG;
 }

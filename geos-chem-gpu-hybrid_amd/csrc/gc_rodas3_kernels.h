@@ -27,7 +27,40 @@ struct GcBatchArgs {
 struct GcVariantInfo {
   int nvar, nfix, nreact, nnz;
   long ws_doubles_per_slot;
+  long adj_doubles_per_slot;  // workspace of one cell of the adjoint sweep
   int waves_per_eu, seg;
+};
+
+// Pool of accepted-step records kept by the recording form of the persistent
+// kernel for the discrete adjoint: record s holds the state Y(NVAR) at the
+// start of an accepted step, its H and the previous record of the same cell.
+struct GcRecord {
+  double* Y;          // [cap][NVAR]
+  double* H;          // [cap]
+  int* prev;          // [cap], -1 for a cell's first step
+  int* head;          // [nCell] (caller's cell index): last record, -1 if none
+  unsigned* count;    // records taken; > cap means the pool overflowed
+  unsigned cap;
+};
+
+// Arguments of the adjoint sweep (arrays in the caller's Fortran layout).
+struct GcAdjArgs {
+  int nCell;
+  double Tstart, Tend;
+  const double* C;       // (NSPEC, nCell): fixed species are read
+  const double* RCONST;  // (NREACT, nCell)
+  const int* ICNTRL;     // (20, nCell)
+  const int* Perm;       // (nCell), 1-based launch order
+  const double* recY;
+  const double* recH;
+  const int* recPrev;
+  const int* head;
+  unsigned cap;          // records in the pool (every slot followed is checked)
+  const double* lamC;    // (NSPEC, nCell): dL/dC at Tend
+  double* gC;            // (NSPEC, nCell): dL/dC at Tstart
+  double* gRCONST;       // (NREACT, nCell): dL/dRCONST
+  int* IERR;             // (nCell): 1, -8 singular step, -3 non-autonomous,
+                         // -5 a record index outside the pool
 };
 
 // Internal to the library (not exported): the C-ABI in include/gcchemgpu.h is
@@ -40,6 +73,12 @@ struct GcVariantInfo {
                                                    unsigned S, unsigned* queue,       \
                                                    int block, int cells_per_wave,     \
                                                    int grain, hipStream_t st);        \
+  extern "C" GC_INTERNAL hipError_t gc_launch_record_v##V(                            \
+      const GcBatchArgs* a, double* ws, unsigned S, unsigned* queue, int block,       \
+      int cells_per_wave, int grain, const GcRecord* rec, hipStream_t st);            \
+  extern "C" GC_INTERNAL hipError_t gc_launch_adjoint_v##V(const GcAdjArgs* a,          \
+                                                         double* ws, int block,       \
+                                                         hipStream_t st);             \
   extern "C" GC_INTERNAL void gc_variant_info_v##V(GcVariantInfo* d);
 
 GC_DECLARE_VARIANT(1)

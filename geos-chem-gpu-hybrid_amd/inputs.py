@@ -184,6 +184,17 @@ def config2(mech: Mechanism, seed: int = 2) -> dict:
     return grid_config(mech, 144, 91, 72, seed)
 
 
+def config3(mech: Mechanism, n: int = 65536, seed: int = 3) -> dict:
+    """Adjoint workload: n cells taken evenly across the 4x5 grid generator's
+    238,464 cells (all levels and latitudes represented), plus the loss
+    weights ``w[NVAR]`` ~ N(0, 1) per species, all from `seed`."""
+    d = grid_config(mech, 72, 46, 72, seed)
+    idx = np.linspace(0, d["C"].shape[0] - 1, n).round().astype(np.int64)
+    out = subset(d, idx)
+    out["w"] = np.random.default_rng(seed + 1000).standard_normal(mech.nvar)
+    return out
+
+
 def subset(d: dict, idx) -> dict:
     """Cells `idx` of a generated batch (identity launch order)."""
     idx = np.asarray(idx)

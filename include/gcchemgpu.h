@@ -113,6 +113,45 @@ int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave,
 void gc_amd_get_tuning(int *block, int *blocks_per_cu, int *cells_per_wave,
                        int *grain, int *variant);
 
+/* ---- discrete adjoint (frozen step sizes) -------------------------------- */
+
+/* Forward batch solve (persistent kernel, same arguments and results as
+ * gc_amd_integrate_batch_device with kernel = persistent and chunk = 0) that
+ * also records every accepted step for the adjoint: record s holds the state
+ * at the start of the step, recY[s*NVAR .. s*NVAR+NVAR), its step size recH[s]
+ * and the previous record of the same cell recPrev[s] (-1 for the first);
+ * head[c] (c = 0-based column of the caller's arrays) is the cell's last
+ * record or -1.  The pool holds `cap` records; *used receives the number of
+ * accepted steps.  Returns 0, -4 if *used > cap (the solve results are valid,
+ * the records are not: call again with cap >= *used), or the codes of
+ * gc_amd_integrate_batch_device.  Blocking; ordered after `stream`. */
+int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double *C,
+                               const double *RCONST, const double *ATOL,
+                               const double *RTOL, const int *ICNTRL,
+                               const double *RCNTRL, const int *Perm, int *ISTAT,
+                               double *RSTAT, int *IERR, double *recY, double *recH,
+                               int *recPrev, int *head, long cap, long *used,
+                               void *stream, double *tKernel);
+
+/* Reverse sweep of the discrete adjoint of the recorded accepted Rodas3 steps
+ * (step sizes held fixed).  Given lamC(NSPEC,nCell) = dL/dC at Tend, writes
+ * gC(NSPEC,nCell) = dL/dC at Tstart (VAR and FIX) and gRCONST(NREACT,nCell) =
+ * dL/dRCONST.  C supplies the fixed species; ICNTRL(1) must be non-zero
+ * (autonomous, as GEOS-Chem sets it) and ICNTRL(16) = 1 (clamp) is honoured.
+ * The record arrays are those of a successful gc_amd_record_batch_device
+ * call with pool size `cap`.  IERR(c) = 1, -3 for a non-autonomous cell
+ * (gradients 0), -8 if a recorded step's matrix is singular, -5 if the cell's
+ * chain leaves the pool.  Returns 0 or a negative code as above.  Each
+ * step's factors and stages are recomputed from its recorded start state;
+ * the transposed solves reuse them (KppSolveTR order). */
+int gc_amd_adjoint_batch_device(int nCell, double Tstart, double Tend,
+                                const double *C, const double *RCONST,
+                                const int *ICNTRL, const int *Perm,
+                                const double *recY, const double *recH,
+                                const int *recPrev, const int *head, long cap,
+                                const double *lamC, double *gC, double *gRCONST,
+                                int *IERR, void *stream, double *tKernel);
+
 /* Dimensions of the mechanism compiled into this library. */
 void gc_amd_mech_info(int *nvar, int *nfix, int *nreact, int *lu_nonzero);
 

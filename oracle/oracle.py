@@ -132,6 +132,8 @@ class Oracle:
         L.or_decomp.argtypes = [P, P, P, P]
         L.or_solve.argtypes = [P, P, P]
         L.or_solve_tr.argtypes = [P, P, P, P]
+        L.or_replay_cell.argtypes = [P, P, P, P, D, I, P, I, P, P]
+        L.or_solve_cell_rec.argtypes = [P, P, P, P, D, D, P, P, P, P, P, P, P, P, P, I, P]
         L.or_integrate_batch.restype = D
         L.or_integrate_batch.argtypes = [P, I, D, D, P, P, P, P, P, P, P, P, P, P, I]
 
@@ -172,6 +174,37 @@ class Oracle:
         xx = np.zeros(self.nvar)
         self.lib.or_solve_tr(self._m, _ptr(lu), _ptr(x), _ptr(xx))
         return xx
+
+    def solve_cell_rec(self, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, maxrec=100000):
+        """One cell (C[NSPEC], RCONST[NREACT], ICNTRL/RCNTRL[20]) with the
+        accepted step sizes recorded.  Returns (C, ISTAT, RSTAT, IERR, Hs)."""
+        C = _f64(C).copy()
+        RCONST, ATOL, RTOL = _f64(RCONST), _f64(ATOL), _f64(RTOL)
+        ICNTRL, RCNTRL = _i32(ICNTRL), _f64(RCNTRL)
+        ISTAT, RSTAT = np.zeros(8, np.int32), np.zeros(3)
+        IERR, nrec = ctypes.c_int(0), ctypes.c_int(0)
+        Hrec = np.zeros(maxrec)
+        work = np.zeros(self.lib.or_work_doubles(self._m))
+        F = C[self.nvar:].copy()
+        self.lib.or_solve_cell_rec(self._m, _ptr(C), _ptr(F), _ptr(RCONST), ctypes.c_double(Tstart),
+                                   ctypes.c_double(Tend), _ptr(ATOL), _ptr(RTOL), _ptr(ICNTRL),
+                                   _ptr(RCNTRL), _ptr(ISTAT), _ptr(RSTAT), ctypes.byref(IERR),
+                                   _ptr(work), _ptr(Hrec), int(maxrec), ctypes.byref(nrec))
+        return C, ISTAT, RSTAT, IERR.value, Hrec[:min(nrec.value, maxrec)].copy()
+
+    def replay(self, V, F, RCT, Hs, Dir=1.0, clamp=False):
+        """Final VAR concentrations after the accepted steps of sizes Hs
+        (frozen step-size control).  Raises if a step's matrix is singular."""
+        Y = _f64(V).copy()
+        F, RCT, Hs = _f64(F), _f64(RCT), _f64(Hs)
+        work = np.zeros(self.lib.or_work_doubles(self._m))
+        ier = ctypes.c_int(0)
+        self.lib.or_replay_cell(self._m, _ptr(Y), _ptr(F), _ptr(RCT), ctypes.c_double(Dir),
+                                int(Hs.size), _ptr(Hs), int(bool(clamp)), _ptr(work),
+                                ctypes.byref(ier))
+        if ier.value:
+            raise RuntimeError(f"replay: singular matrix at step {ier.value - 1}")
+        return Y
 
     def integrate_batch(self, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL,
                         Perm=None, nthreads=0):

@@ -169,6 +169,21 @@ void or_solve_cell(const or_mech *m, double *Y, const double *FIX,
                    const int *ICNTRL_U, const double *RCNTRL_U,
                    int *ISTAT, double *RSTAT, int *IERR, double *work)
 {
+    or_solve_cell_rec(m, Y, FIX, RCONST, Tstart, Tend, AbsTol, RelTol, ICNTRL_U,
+                      RCNTRL_U, ISTAT, RSTAT, IERR, work, NULL, 0, NULL);
+}
+
+/* As or_solve_cell; additionally the step sizes H of the accepted steps (the
+ * stepper's own H, sign included), in order, go to Hrec[0..maxrec) and their
+ * number to *nrec (when Hrec is not NULL). */
+void or_solve_cell_rec(const or_mech *m, double *Y, const double *FIX,
+                       const double *RCONST, double Tstart, double Tend,
+                       const double *AbsTol, const double *RelTol,
+                       const int *ICNTRL_U, const double *RCNTRL_U,
+                       int *ISTAT, double *RSTAT, int *IERR, double *work,
+                       double *Hrec, int maxrec, int *nrec)
+{
+    if (Hrec) *nrec = 0;
     static const double ros_A[6] = {0.0, 2.0, 0.0, 2.0, 0.0, 1.0};
     static const double ros_C[6] = {4.0, 1.0, -1.0, 1.0, -1.0, -(8.0 / 3.0)};
     static const int ros_NewF[4] = {1, 0, 1, 1};
@@ -322,6 +337,10 @@ void or_solve_cell(const or_mech *m, double *Y, const double *FIX,
             ISTAT[2] += 1;
             if ((Err <= 1.0) || (H <= Hmin)) {
                 ISTAT[3] += 1;
+                if (Hrec) {
+                    if (*nrec < maxrec) Hrec[*nrec] = H;
+                    *nrec += 1;
+                }
                 if (ICNTRL[15] == 1) {
                     for (int k = 0; k < NVAR; ++k) Y[k] = fmax_f(Ynew[k], 0.0);
                 } else {
@@ -347,6 +366,69 @@ void or_solve_cell(const or_mech *m, double *Y, const double *FIX,
         }
     }
     *IERR = 1;
+}
+
+/* The accepted steps of Integrate_Cell_S with their step sizes given: the
+ * discrete map that the Rodas3 discrete adjoint differentiates (step sizes and
+ * the accept/reject sequence held fixed).  Autonomous form (ICNTRL(1) /= 0,
+ * the GEOS-Chem setting); each step is the stage sequence, the prepared matrix
+ * and the update of gckpp_BatchIntegrator.F90:565-690 with the stepper's H =
+ * Hs[k] (as recorded by or_solve_cell_rec), clamped at 0 when clamp is set.
+ * *ier = 0, or k+1 if the matrix of step k is singular. */
+void or_replay_cell(const or_mech *m, double *Y, const double *FIX,
+                    const double *RCONST, double Dir, int nsteps,
+                    const double *Hs, int clamp, double *work, int *ier)
+{
+    static const double ros_A[6] = {0.0, 2.0, 0.0, 2.0, 0.0, 1.0};
+    static const double ros_C[6] = {4.0, 1.0, -1.0, 1.0, -1.0, -(8.0 / 3.0)};
+    static const double ros_M[4] = {2.0, 0.0, 1.0, 1.0};
+    static const double ros_Gamma[4] = {0.5, 1.5, 0.0, 0.0};
+    const int NVAR = m->nvar, NNZ = m->nnz;
+    double *Kw = work;
+    double *Ghimj = Kw + 4 * NVAR;
+    double *Ynew = Ghimj + NNZ;
+    double *Fcn0 = Ynew + NVAR;
+    double *Fcn = Fcn0 + NVAR;
+    double *wP = Fcn + NVAR;
+    double *wD = wP + NVAR;
+    double *A = wD + NVAR;
+    double *B = A + m->nreact;
+    double *W = B + m->nb;
+    *ier = 0;
+    for (int st = 0; st < nsteps; ++st) {
+        const double H = Hs[st];
+        or_fun_split(m, Y, FIX, RCONST, Fcn0, wP, wD, A);
+        or_jac_sp(m, Y, FIX, RCONST, Ghimj, B);
+        for (int k = 0; k < NNZ; ++k) Ghimj[k] = -Ghimj[k];
+        const double ghinv = 1.0 / ((Dir * H) * ros_Gamma[0]);
+        for (int i = 0; i < NVAR; ++i) Ghimj[m->diag[i]] = Ghimj[m->diag[i]] + ghinv;
+        int ising;
+        or_decomp(m, Ghimj, W, &ising);
+        if (ising) { *ier = st + 1; return; }
+        for (int istage = 1; istage <= 4; ++istage) {
+            double *Ks = Kw + NVAR * (istage - 1);
+            if (istage == 1 || istage == 2) {
+                for (int k = 0; k < NVAR; ++k) Fcn[k] = Fcn0[k];
+            } else {
+                for (int k = 0; k < NVAR; ++k) Ynew[k] = Y[k];
+                for (int j = 1; j <= istage - 1; ++j) {
+                    double aa = ros_A[(istage - 1) * (istage - 2) / 2 + j - 1];
+                    for (int k = 0; k < NVAR; ++k) Ynew[k] = Ynew[k] + aa * Kw[NVAR * (j - 1) + k];
+                }
+                or_fun_split(m, Ynew, FIX, RCONST, Fcn, wP, wD, A);
+            }
+            for (int k = 0; k < NVAR; ++k) Ks[k] = Fcn[k];
+            for (int j = 1; j <= istage - 1; ++j) {
+                double HC = ros_C[(istage - 1) * (istage - 2) / 2 + j - 1] / (Dir * H);
+                for (int k = 0; k < NVAR; ++k) Ks[k] = Ks[k] + HC * Kw[NVAR * (j - 1) + k];
+            }
+            or_solve(m, Ghimj, Ks);
+        }
+        for (int k = 0; k < NVAR; ++k) Ynew[k] = Y[k];
+        for (int j = 0; j < 4; ++j)
+            for (int k = 0; k < NVAR; ++k) Ynew[k] = Ynew[k] + ros_M[j] * Kw[NVAR * j + k];
+        for (int k = 0; k < NVAR; ++k) Y[k] = clamp ? fmax_f(Ynew[k], 0.0) : Ynew[k];
+    }
 }
 
 /* gc_gpu_integrate_batch_dev semantics on the CPU (gc_gpu_batch_api.F90:

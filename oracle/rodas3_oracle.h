@@ -65,6 +65,21 @@ void or_solve_cell(const or_mech *m, double *Y, const double *FIX,
                    const int *ICNTRL_U, const double *RCNTRL_U,
                    int *ISTAT, double *RSTAT, int *IERR, double *work);
 
+/* As or_solve_cell, recording the accepted step sizes (the stepper's H) in
+ * Hrec[0..maxrec) and their count in *nrec. */
+void or_solve_cell_rec(const or_mech *m, double *Y, const double *FIX,
+                       const double *RCONST, double Tstart, double Tend,
+                       const double *ATOL, const double *RTOL,
+                       const int *ICNTRL_U, const double *RCNTRL_U,
+                       int *ISTAT, double *RSTAT, int *IERR, double *work,
+                       double *Hrec, int maxrec, int *nrec);
+
+/* Accepted steps with given step sizes Hs[0..nsteps) (frozen step-size
+ * control; autonomous).  *ier = 0 or 1 + the index of a singular step. */
+void or_replay_cell(const or_mech *m, double *Y, const double *FIX,
+                    const double *RCONST, double Dir, int nsteps,
+                    const double *Hs, int clamp, double *work, int *ier);
+
 /* Batched driver with the gc_gpu_integrate_batch argument layout.
  * nthreads <= 0 uses the OpenMP default. Returns wall seconds of the solve. */
 double or_integrate_batch(const or_mech *m, int nCell, double Tstart,

@@ -1,0 +1,182 @@
+"""GPU checks of the recording forward solve and the Rodas3 discrete adjoint.
+
+* Recording changes nothing in the forward results, keeps exactly one record
+  per accepted step, and the recorded step sizes replayed by the oracle from
+  the initial state reproduce the GPU's final concentrations bit for bit.
+* The GPU adjoint equals the NumPy reference adjoint (tests/adjoint_ref.py,
+  stoichiometry from the reference's CalcStoichNum) to summation-order
+  rounding, and central finite differences of the oracle replay (relative
+  step 1e-6) on 256 (cell, parameter) pairs to 1e-6 relative above the
+  rounding noise of the loss.
+* The torch.autograd.Function returns the same gradients.
+"""
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+import torch
+
+from adjoint_ref import Monomials, adjoint_cell, log_loss, stoich_matrix
+from gcchem_amd import adjoint, inputs, solver
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def adj_case(mech):
+    d = inputs.config3(mech, n=1024, seed=3)
+    b = solver.Batch.from_numpy(d)
+    _say("record solve: config3 generator, 1024 cells")
+    out, rec = adjoint.record_solve(b)
+    torch.cuda.synchronize()
+    return d, b, out, rec
+
+
+def test_record_solve_matches_forward(mech, oracle_cr, adj_case):
+    d, b, out, rec = adj_case
+    ref = solver.solve(b, kernel="persistent")
+    torch.cuda.synchronize()
+    for k in ("C", "ISTAT", "RSTAT", "IERR"):
+        assert torch.equal(getattr(out, k), getattr(ref, k)), k
+    nacc = out.ISTAT[:, 3].cpu().numpy()
+    assert rec.used == int(nacc.sum())
+    rng = np.random.default_rng(1)
+    for c in rng.choice(b.n, 16, replace=False):
+        Hs = adjoint.records_of_cell(rec, int(c))
+        assert Hs.size == nacc[c]
+        Y = oracle_cr.replay(d["C"][c][:mech.nvar], d["C"][c][mech.nvar:], d["RCONST"][c], Hs)
+        g = out.C[c, :mech.nvar].cpu().numpy()
+        assert np.array_equal(Y.view(np.int64), g.view(np.int64)), int(c)
+
+
+def test_record_pool_regrows(mech):
+    d = inputs.config0(mech, n=300, seed=8)
+    b = solver.Batch.from_numpy(d)
+    _say("record solve with a 10-record pool, 300 cells")
+    out, rec = adjoint.record_solve(b, cap=10)
+    ref = solver.solve(b, kernel="persistent")
+    torch.cuda.synchronize()
+    assert torch.equal(out.C, ref.C)
+    assert rec.used == int(out.ISTAT[:, 3].sum().item()) and rec.Y.shape[0] >= rec.used
+
+
+@pytest.fixture(scope="module")
+def adj_grads(mech, adj_case):
+    d, b, out, rec = adj_case
+    C = out.C.cpu().numpy()
+    _, lamV = log_loss(C[:, :mech.nvar], d["w"][None, :], d["ATOL"][None, :])
+    lam = np.zeros_like(C)
+    lam[:, :mech.nvar] = lamV
+    _say("adjoint sweep: 1024 cells")
+    gC, gR, ierr, tk = adjoint.adjoint_sweep(b, rec, torch.as_tensor(lam, device="cuda"))
+    torch.cuda.synchronize()
+    assert bool((ierr == 1).all())
+    return lam, gC.cpu().numpy(), gR.cpu().numpy()
+
+
+def test_adjoint_matches_reference(mech, oracle_cr, checkforms, adj_case, adj_grads):
+    d, b, out, rec = adj_case
+    lam, gC, gR = adj_grads
+    mono = Monomials(mech.ir)
+    S = stoich_matrix(checkforms, mech.nvar, mech.nreact)
+    nv = mech.nvar
+    for c in (0, 100, 517, 1023):
+        Hs = adjoint.records_of_cell(rec, c)
+        rV, rF, rK = adjoint_cell(oracle_cr, mono, S, d["C"][c][:nv], d["C"][c][nv:],
+                                  d["RCONST"][c], Hs, lam[c][:nv], lam[c][nv:])
+        for got, ref in ((gC[c][:nv], rV), (gC[c][nv:], rF), (gR[c], rK)):
+            scale = np.max(np.abs(ref)) + 1e-300
+            assert np.max(np.abs(got - ref)) <= 1e-8 * scale, c
+            rel = np.abs(got - ref) / (np.abs(ref) + 1e-300 * scale)
+            assert np.mean(rel <= 1e-9) >= 0.95 or ref.size < 10, c
+
+
+def test_adjoint_finite_differences_256_pairs(mech, oracle_libm, adj_case, adj_grads):
+    """Central differences of the frozen-step replay, relative step 1e-6, on
+    256 (cell, parameter) pairs over initial VAR concentrations, fixed species
+    and RCONST: |g - fd| <= 1e-6 |fd| + the rounding noise of the loss."""
+    d, b, out, rec = adj_case
+    lam, gC, gR = adj_grads
+    nv = mech.nvar
+    rng = np.random.default_rng(256)
+    cells = rng.choice(b.n, 64, replace=False)
+    pairs = []
+    for c in cells:
+        V, F, R = d["C"][c][:nv], d["C"][c][nv:], d["RCONST"][c]
+        g = {"V": gC[c][:nv], "F": gC[c][nv:], "RCT": gR[c]}
+        vals = {"V": V, "F": F, "RCT": R}
+        for which, n in (("V", 2), ("RCT", 1), ("F", 1)):
+            ok = np.nonzero(vals[which] != 0.0)[0]
+            sens = np.abs(g[which][ok] * vals[which][ok])
+            j = ok[np.argmax(sens)] if rng.uniform() < 0.5 else rng.choice(ok)
+            pairs.append((int(c), which, int(j)))
+            if n == 2:
+                pairs.append((int(c), which, int(rng.choice(ok))))
+    pairs = pairs[:256]
+    assert len(pairs) == 256
+    Hs = {c: adjoint.records_of_cell(rec, c) for c in set(p[0] for p in pairs)}
+
+    def one(p):
+        c, which, j = p
+        args = {"V": d["C"][c][:nv].copy(), "F": d["C"][c][nv:].copy(), "RCT": d["RCONST"][c].copy()}
+        x = args[which][j]
+        h = 1e-6 * abs(x)
+
+        def L(v):
+            a = {k: v_.copy() for k, v_ in args.items()}
+            a[which][j] = v
+            Y = oracle_libm.replay(a["V"], a["F"], a["RCT"], Hs[c])
+            return log_loss(Y, d["w"], d["ATOL"])[0]
+        fd = (L(x + h) - L(x - h)) / (2 * h)
+        L0 = L(x)
+        # rounding noise of the loss after ~100 stiff steps: 256 ulps of its
+        # term magnitude (FD at larger steps converges to the adjoint where this
+        # noise dominates, i.e. for gradients far below the cell's largest)
+        noise = 2.0 ** -44 * (abs(L0) + np.sum(np.abs(d["w"])) * 40.0) / (2 * h)
+        g = {"V": gC[c][:nv], "F": gC[c][nv:], "RCT": gR[c]}[which][j]
+        return abs(g - fd) <= 1e-6 * abs(fd) + noise, (c, which, j, g, fd, noise)
+
+    _say("finite differences: 256 pairs on the oracle replay")
+    with ThreadPoolExecutor(8) as ex:
+        res = list(ex.map(one, pairs))
+    bad = [info for ok, info in res if not ok]
+    assert not bad, bad[:5]
+
+
+def test_autograd_function(mech, adj_case, adj_grads):
+    d, b, out, rec = adj_case
+    lam, gC, gR = adj_grads
+    C = b.C.clone().requires_grad_(True)
+    R = b.RCONST.clone().requires_grad_(True)
+    _say("autograd: Rodas3Solve forward + backward, 1024 cells")
+    Cf = adjoint.solve(C, R, b)
+    assert torch.equal(Cf.detach(), out.C)
+    w = torch.as_tensor(d["w"], device="cuda")
+    at = torch.as_tensor(d["ATOL"], device="cuda")
+    x = Cf[:, :mech.nvar]
+    loss = (w * 0.5 * torch.log(x * x + at * at)).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    g = C.grad.cpu().numpy()
+    scale = np.abs(gC).max()
+    assert np.max(np.abs(g - gC)) <= 1e-12 * scale
+    assert np.max(np.abs(R.grad.cpu().numpy() - gR)) <= 1e-12 * np.abs(gR).max()
+
+
+def test_adjoint_non_autonomous_flagged(mech):
+    d = inputs.config0(mech, n=70, seed=2)
+    d["ICNTRL"][:, 0] = 0
+    d["ICNTRL"][:5, 0] = 1
+    b = solver.Batch.from_numpy(d)
+    _say("adjoint of a partly non-autonomous batch, 70 cells")
+    out, rec = adjoint.record_solve(b)
+    lam = torch.ones_like(b.C)
+    gC, gR, ierr, _ = adjoint.adjoint_sweep(b, rec, lam)
+    torch.cuda.synchronize()
+    ie = ierr.cpu().numpy()
+    assert (ie[:5] == 1).all() and (ie[5:] == -3).all()
+    assert float(gC[5:].abs().sum()) == 0.0

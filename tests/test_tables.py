@@ -46,8 +46,11 @@ class TableMachine:
         self.lu_ops = self.t.lu(seg)
         self.fwd = [self.t.fwd("K", s) for s in range(4)]
         self.bwd = [self.t.bwd("K", s) for s in range(4)]
+        self.tr = self.t.tr("TMP")
+        self.nu_tab = self.t.adj_nu()
+        self.mono = {"Y": self.t.adj_mono("Y"), "YNEW": self.t.adj_mono("YNEW")}
         self.lits = np.array(self.t.lits or [0.0])
-        self.ws = np.full((self.E["TOTAL"], lanes), np.nan)
+        self.ws = np.full((self.E["ADJ_TOTAL"], lanes), np.nan)
         self.ws[self.E["ZERO"]] = 0.0
         self.lds = np.full((seg + 1, lanes), np.nan)
 
@@ -162,6 +165,137 @@ class TableMachine:
         self.solve_rows(self.fwd[s], False)
         self.solve_rows(self.bwd[s], True)
 
+    # -- adjoint routines, as adj_* in gc_rodas3.hip ---------------------------
+    ROS_A = (0.0, 2.0, 0.0, 2.0, 0.0, 1.0)
+    ROS_C = (4.0, 1.0, -1.0, 1.0, -1.0, -(8.0 / 3.0))
+    ROS_M = (2.0, 0.0, 1.0, 1.0)
+
+    def o(self, name, k=0):
+        return (self.E[name] + k) * 512
+
+    def solve_tr(self):
+        w1, n1, w2, n2 = self.tr
+        self.solve_rows((w1, n1), True)
+        self.solve_rows((w2, n2), False)
+
+    def stage_point(self, s):
+        n = self.ir["nvar"]
+        for k in range(n):
+            y = self.ld(self.o("Y", k)).copy()
+            for j in range(s):
+                y = y + self.ROS_A[s * (s - 1) // 2 + j] * self.ld(self.o("K", j * n + k))
+            self.st(self.o("YNEW", k), y)
+
+    def stages(self, dh):
+        """Stage vectors; dh (scalar or per lane) is Dir*H."""
+        n = self.ir["nvar"]
+        for s in range(4):
+            Ks = self.o("K", s * n)
+            if s < 2:
+                for k in range(n):
+                    self.st(Ks + k * 512, self.ld(self.o("FCN0", k)).copy())
+            else:
+                self.stage_point(s)
+                self.fun(True, Ks)
+            for j in range(s):
+                hc = self.ROS_C[s * (s - 1) // 2 + j] / dh
+                for k in range(n):
+                    self.st(Ks + k * 512, self.ld(Ks + k * 512) + hc * self.ld(self.o("K", j * n + k)))
+            self.solve(s)
+
+    def adj_nu(self):
+        w, p = self.nu_tab, 0
+        for _ in range(self.ir["nreact"]):
+            n, dst = w[p], w[p + 1]
+            p += 4
+            if n:
+                v, p = self.sum_terms(w, p, n)
+            else:
+                v = np.zeros(self.ws.shape[1])
+            self.st(dst, v)
+        assert p == len(w)
+
+    def acc(self, g, v):
+        self.st(g, self.ld(g) + v)
+
+    def adj_mono(self, second, point):
+        q = self.mono[point]
+        for r in range(self.ir["nreact"]):
+            rec = q[16 * r:16 * r + 16]
+            nf, hk, li = rec[0], rec[1], rec[2]
+            nu = self.ld(rec[3])
+            c = self.lits[li] if li >= 0 else 1.0
+            k = self.ld(rec[4]) if hk else 1.0
+            x1 = self.ld(rec[6])
+            w = nu * c * k
+            if not second:
+                if nf == 1:
+                    self.acc(rec[12], w)
+                    if hk: self.acc(rec[5], nu * c * x1)
+                elif nf == 2:
+                    x2 = self.ld(rec[7])
+                    self.acc(rec[12], w * x2)
+                    self.acc(rec[13], w * x1)
+                    if hk: self.acc(rec[5], nu * c * (x1 * x2))
+                else:
+                    x2, x3 = self.ld(rec[7]), self.ld(rec[8])
+                    self.acc(rec[12], w * (x2 * x3))
+                    self.acc(rec[13], w * (x1 * x3))
+                    self.acc(rec[14], w * (x1 * x2))
+                    if hk: self.acc(rec[5], nu * c * (x1 * x2 * x3))
+            else:
+                d1 = self.ld(rec[9])
+                if nf == 1:
+                    if hk: self.acc(rec[5], nu * c * d1)
+                elif nf == 2:
+                    x2, d2 = self.ld(rec[7]), self.ld(rec[10])
+                    self.acc(rec[12], w * d2)
+                    self.acc(rec[13], w * d1)
+                    if hk: self.acc(rec[5], nu * c * (x2 * d1 + x1 * d2))
+                else:
+                    x2, x3 = self.ld(rec[7]), self.ld(rec[8])
+                    d2, d3 = self.ld(rec[10]), self.ld(rec[11])
+                    self.acc(rec[12], w * (x3 * d2 + x2 * d3))
+                    self.acc(rec[13], w * (x3 * d1 + x1 * d3))
+                    self.acc(rec[14], w * (x2 * d1 + x1 * d2))
+                    if hk: self.acc(rec[5], nu * c * (x2 * x3 * d1 + x1 * x3 * d2 + x1 * x2 * d3))
+
+    def adj_step(self, dh):
+        """Reverse sweep of one step; dh (scalar or per lane) is Dir*H."""
+        n = self.ir["nvar"]
+        for k in range(n):
+            l = self.ld(self.o("LAM", k)).copy()
+            for j in range(4):
+                self.st(self.o("LK", j * n + k), self.ROS_M[j] * l)
+            self.st(self.o("GY", k), l)
+        for s in (3, 2, 1, 0):
+            for k in range(n):
+                self.st(self.o("TMP", k), self.ld(self.o("LK", s * n + k)).copy())
+            self.solve_tr()
+            self.adj_nu()
+            if s >= 2:
+                self.stage_point(s)
+            for k in range(n):
+                self.st(self.o("LP", k), 0.0 * self.ld(self.o("ZERO")))
+            self.adj_mono(False, "YNEW" if s >= 2 else "Y")
+            for k in range(n):
+                lp, mu = self.ld(self.o("LP", k)), self.ld(self.o("TMP", k))
+                self.acc(self.o("GY", k), lp)
+                for j in range(s):
+                    o = self.o("LK", j * n + k)
+                    v = self.ld(o) + (self.ROS_C[s * (s - 1) // 2 + j] / dh) * mu
+                    if s >= 2:
+                        v = v + self.ROS_A[s * (s - 1) // 2 + j] * lp
+                    self.st(o, v)
+            for k in range(n):
+                self.st(self.o("DIR", k), self.ld(self.o("K", s * n + k)).copy())
+                self.st(self.o("LP", k), 0.0 * self.ld(self.o("ZERO")))
+            self.adj_mono(True, "Y")
+            for k in range(n):
+                self.acc(self.o("GY", k), self.ld(self.o("LP", k)))
+        for k in range(n):
+            self.st(self.o("LAM", k), self.ld(self.o("GY", k)).copy())
+
     def vec(self, name, k=0):
         n = self.ir["nvar"]
         e = self.E[name] + k * n
@@ -252,3 +386,74 @@ def test_padding_is_exact_noop():
     x = np.array([-0.0, 0.0, 1e-300, -3.5])
     y = x - np.float64(0.0) * np.float64(0.0)
     assert _same(y, x)
+
+
+@pytest.mark.parametrize("name", ["fullchem", "stress500"])
+def test_adjoint_step_tables_match_reference(name, checkforms):
+    """One adjoint step through the generated tables (the kernel's loops)
+    equals the NumPy reference adjoint of the same step, at the start states
+    and step sizes of accepted steps of seeded cells."""
+    from adjoint_ref import Monomials, adjoint_cell, stoich_matrix
+    from gcchem_amd import inputs, mechanism
+    from oracle import Oracle
+    mech = mechanism.load(name)
+    orc = Oracle(mech.ir, pow_mode="cr")
+    ir = mech.ir
+    n, nf, nr = ir["nvar"], ir["nfix"], ir["nreact"]
+    d = inputs.config0(mech, n=3, seed=5) if name == "fullchem" else inputs.config4(mech, 3, seed=5)
+    if name != "fullchem":
+        d["Tend"] = 20.0
+    lanes = 3
+    V, F, R, Hs = np.zeros((lanes, n)), d["C"][:, n:].copy(), d["RCONST"].copy(), []
+    for c in range(lanes):
+        *_, hrec = orc.solve_cell_rec(d["Tstart"], d["Tend"], d["C"][c], d["RCONST"][c],
+                                      d["ATOL"], d["RTOL"], d["ICNTRL"][c], d["RCNTRL"][c])
+        k = len(hrec) // 2
+        V[c] = orc.replay(d["C"][c][:n], F[c], R[c], hrec[:k])
+        Hs.append(hrec[k])
+    Hs = np.array(Hs)
+    rng = np.random.default_rng(21)
+    lam = rng.standard_normal((lanes, n))
+    lamF = rng.standard_normal((lanes, nf))
+    m = TableMachine(ir, lanes, 16)
+    _load(m, V, F, R)
+    E = m.E
+    m.ws[E["LAM"]:E["LAM"] + n] = lam.T
+    m.ws[E["GF"]:E["GF"] + nf] = lamF.T
+    m.ws[E["GK"]:E["GK"] + nr] = 0.0
+    m.fun(False, m.o("FCN0"))
+    assert not m.jac_lu(1.0 / (Hs * 0.5)).any()
+    m.stages(Hs)
+    m.adj_step(Hs)
+    gV = m.ws[E["LAM"]:E["LAM"] + n].T
+    gF = m.ws[E["GF"]:E["GF"] + nf].T
+    gK = m.ws[E["GK"]:E["GK"] + nr].T
+    # reference stoichiometry: the reference's CalcStoichNum for fullchem, the
+    # generator's for the synthetic mechanism (checked separately below)
+    if name == "fullchem":
+        S = stoich_matrix(checkforms, n, nr)
+    else:
+        S = np.zeros((n, nr))
+        for i, row in enumerate(GT.Tables(ir).stoichiometry()):
+            for r, c in row.items():
+                S[i, r] = c
+    mono = Monomials(ir)
+    # Same sums in different orders: every entry to 1e-9 of its vector's
+    # scale and almost all to 1e-10 relative; a wrong offset or a dropped term
+    # breaks both by orders of magnitude.
+    for c in range(lanes):
+        rV, rF, rK = adjoint_cell(orc, mono, S, V[c], F[c], R[c], [Hs[c]], lam[c], lamF[c])
+        for got, ref in ((gV[c], rV), (gF[c], rF), (gK[c], rK)):
+            scale = np.max(np.abs(ref)) + 1e-300
+            assert np.max(np.abs(got - ref)) <= 1e-9 * scale
+            rel = np.abs(got - ref) / (np.abs(ref) + 1e-300 * scale)
+            assert np.mean(rel <= 1e-10) >= 0.95 or ref.size < 10
+
+
+def test_stoichiometry_matches_calcstoichnum(mech, checkforms):
+    """The net stoichiometry the generator derives from the Fun_SPLIT sums is
+    the reference's CalcStoichNum, entry for entry."""
+    S = GT.Tables(mech.ir).stoichiometry()
+    mine = {(i, r): c for i, row in enumerate(S) for r, c in row.items() if c != 0.0}
+    ref = {(i, r): float(v) for i, r, v in checkforms["stoich"]}
+    assert mine == ref
