@@ -10,7 +10,9 @@ of src/geos-chem/GeosCore/fullchem_mod.F90:905-981.
 
 One "step" is one batch solve of all cells with the inputs resident in HBM.
 `value` is the persistent kernel's cells/s; the plain one-thread-per-cell
-kernel is timed on the same cells in the same order for the ratio.  The CPU
+kernel is timed on the same cells in the same order for the ratio.  The
+adjoint/forward cost ratio is measured on 65,536 cells of the same generator
+(recording forward + reverse sweep over the forward solve).  The CPU
 baseline is the oracle (a plain-C restatement of the reference Rodas3, libm
 pow) on a bounded sample of the same cells with OpenMP on this host.
 
@@ -76,6 +78,43 @@ def cpu_baseline(mech, d, perm, target_s=15.0):
                       f"{secs:.1f} s wall"}
 
 
+def adjoint_ratio(mech, n=65536):
+    """Adjoint workload: n cells of the 4x5 grid generator, loss = weighted
+    log final concentrations.  Forward = persistent solve in cost order;
+    adjoint = recording forward + reverse sweep (HIP-event kernel times)."""
+    import numpy as np
+    import torch
+    from gcchem_amd import adjoint, inputs, solver
+    d = inputs.config3(mech, n=n, seed=3)
+    b = solver.Batch.from_numpy(d)
+    r = solver.solve(b, kernel="persistent")
+    torch.cuda.synchronize()
+    b.Perm = torch.as_tensor(inputs.cost_order(r.ISTAT[:, 2].cpu().numpy()), device="cuda")
+    fwd = []
+    for _ in range(2):
+        r = solver.solve(b, kernel="persistent")
+        torch.cuda.synchronize()
+        fwd.append(r.t_kernel)
+    out, rec = adjoint.record_solve(b)
+    torch.cuda.synchronize()
+    w = torch.as_tensor(d["w"], device="cuda")
+    at = torch.as_tensor(d["ATOL"], device="cuda")
+    x = out.C[:, :mech.nvar]
+    lam = torch.zeros_like(out.C)
+    lam[:, :mech.nvar] = w * x / (x * x + at * at)
+    perm = torch.as_tensor(inputs.cost_order(out.ISTAT[:, 3].cpu().numpy()), device="cuda")
+    gC, gR, ierr, t_sweep = adjoint.adjoint_sweep(b, rec, lam, perm=perm)
+    torch.cuda.synchronize()
+    if not bool((ierr == 1).all()):
+        raise RuntimeError("adjoint sweep failed for some cells")
+    t_fwd = float(np.mean(fwd))
+    log(f"adjoint ({n} cells): forward {t_fwd * 1e3:.1f} ms, record {rec.t_kernel * 1e3:.1f} ms, "
+        f"sweep {t_sweep * 1e3:.1f} ms")
+    return {"cells": n, "forward_s": round(t_fwd, 4), "record_s": round(rec.t_kernel, 4),
+            "sweep_s": round(t_sweep, 4), "records": rec.used,
+            "adj_over_fwd": round((rec.t_kernel + t_sweep) / t_fwd, 3)}
+
+
 def host_api_rate(lib, d, perm):
     """One call of the reference entry point gc_gpu_integrate_batch on host
     arrays (PCIe staging in and out included): (cells/s, tTotal, tKernel)."""
@@ -111,6 +150,58 @@ def copy_probe(torch, gib=1):
     del src, dst
 
 
+def bench_grid_2x25(args, mech, lib, tune):
+    """2x2.5 grid (144 x 91 x 72 = 943,488 cells): the persistent kernel over
+    the whole grid in HBM-sized chunks of 262,144 cells, the chunk loop
+    captured once as a HIP graph (GC_GPU_WORKSPACE=global mode), against the
+    plain kernel over the whole grid in one launch; cost order from a first
+    solve."""
+    import numpy as np
+    import torch
+    from gcchem_amd import inputs, solver
+    t0 = time.time()
+    d = inputs.config2(mech, seed=2)
+    n = d["C"].shape[0]
+    log(f"generated {n} cells in {time.time() - t0:.1f} s")
+    b = solver.Batch.from_numpy(d)
+    del d
+    r = solver.solve(b, kernel="persistent")
+    torch.cuda.synchronize()
+    log(f"previous step (grid order): kernel {r.t_kernel * 1e3:.1f} ms")
+    istat = r.ISTAT.cpu().numpy()
+    b.Perm = torch.as_tensor(inputs.cost_order(istat[:, 2]), device="cuda")
+    chunk = 262144
+    out, tk = None, []
+    for k in range(args.warmup + args.steps):
+        out = solver.solve(b, kernel="persistent", chunk=chunk, out=out)
+        torch.cuda.synchronize()
+        log(f"persistent chunked step {k}: {out.t_kernel * 1e3:.1f} ms")
+        if k >= args.warmup:
+            tk.append(out.t_kernel)
+    ok = float((out.IERR == 1).float().mean().item())
+    rp = solver.solve(b, kernel="plain", out=out)
+    torch.cuda.synchronize()
+    log(f"plain: {rp.t_kernel * 1e3:.1f} ms")
+    t = float(np.mean(tk))
+    flops = mech.algorithmic_flops(istat)
+    return {
+        "metric": METRIC, "value": round(n / t, 1), "unit": "cells/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "fullchem Rodas3, 2x2.5 grid 144x91x72 = 943,488 cells, dt 600 s, "
+                               "cost order, persistent kernel in 262,144-cell chunks replayed "
+                               "as one HIP graph", "cells": n, "chunk": chunk, "tuning": tune},
+        "plain_cells_per_s": round(n / rp.t_kernel, 1),
+        "speedup_vs_plain": round(rp.t_kernel / t, 3),
+        "ierr_ok_frac": ok,
+        "roofline": {"bound": "mfma", "achieved": round(flops / t / 1e12, 4),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(flops / t / 1e12 / FP64_PEAK_TFLOPS, 5), "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +209,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=int, default=0, help="use the first N cells (debug)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    ap.add_argument("--grid", default="4x5", choices=["4x5", "2x2.5"],
+                    help="4x5 (238,464 cells; the headline line) or 2x2.5 (943,488 cells, "
+                         "chunked graph)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB elementwise copy first (counter calibration)")
     args = ap.parse_args()
@@ -137,6 +232,9 @@ def main():
     mech = mechanism.load("fullchem")
     lib = L.load("fullchem")
     tune = tuning.apply(lib, "fullchem")
+    if args.grid == "2x2.5":
+        print(json.dumps(bench_grid_2x25(args, mech, lib, tune)), flush=True)
+        return
     t0 = time.time()
     d = inputs.config1(mech, seed=1)
     if args.cells:
@@ -175,6 +273,7 @@ def main():
     log(f"plain: {s_plain * 1e3:.1f} ms/step, kernel {tk_plain * 1e3:.1f} ms")
 
     host_rate, host_tt, host_tk = host_api_rate(lib, d, perm)
+    adj = adjoint_ratio(mech) if not args.no_adjoint else None
     log(f"host-array entry point: tTotal {host_tt * 1e3:.1f} ms, tKernel {host_tk * 1e3:.1f} ms")
 
     flops = mech.algorithmic_flops(istat)
@@ -212,7 +311,8 @@ def main():
         "s_per_chem_step": round(s_pers, 4),
         "host_api_cells_per_s": round(host_rate, 1),
         "host_api_t_total_s": round(host_tt, 4),
-        "adj_over_fwd": None,
+        "adj_over_fwd": adj["adj_over_fwd"] if adj else None,
+        "adjoint": adj,
         "ierr_ok_frac": float((ierr == 1).mean()),
         "mean_steps_per_cell": float(istat[:, 2].mean()),
         "roofline": roof,

@@ -34,12 +34,12 @@ Tables (int32 words):
   ``prodgrp_<name>`` gives the record count of each of the 8 shapes
   (shape = (nf-1)*2 + has_lit).  Value = lit*f0*f1*... left to right.
   Products: ``rate_y``/``rate_yn`` (A(r), V = Y or Ynew; gckpp_Function.F90:
-  2215-3248), ``dprod_y``/``dprod_yn`` (each D_VAR term's product, stored in
-  the DP scratch), ``bjac`` (Jac_SP's B; gckpp_Jacobian.F90:63-3673).
+  2215-3248), ``bjac`` (Jac_SP's B; gckpp_Jacobian.F90:63-3673).
 * ``pd_y``/``pd_yn``: per species 4-word header ``{nP, nD, V(i), i*512}``
-  then nP + nD 4-word terms ``{soff, 0, coef_lo, coef_hi}``: P_VAR(i) over A,
-  D_VAR(i) over DP, Vdot(i) = P_VAR(i) - D_VAR(i)*V(i).
-  (gckpp_Function.F90:3252-5058)
+  then nP 4-word terms ``{soff, 0, coef_lo, coef_hi}`` (P_VAR(i) over A) and
+  nD 4-word product terms ``{f1, f2, f3, ci}`` (D_VAR(i), each term
+  ((lits[ci]*f1)*f2)*f3 evaluated in place, missing factors reading ONE);
+  Vdot(i) = P_VAR(i) - D_VAR(i)*V(i).  (gckpp_Function.F90:3252-5058)
 * ``lu<S>``: fused Jac_SP JVS sums + matrix preparation + KppDecomp with the
   active row segment (at most S entries, plus one spare slot) in LDS; 4-word
   ops: ENTRY {1, slot, nterms, diag} + nterms terms; PIVL {2, slot_j, U(j,j),
@@ -70,7 +70,6 @@ SEGMENTS = (16, 32)     # LU row-segment sizes compiled (LDS entries per lane)
 
 def layout(ir) -> dict:
     nvar, nfix, nreact, nnz, nb = ir["nvar"], ir["nfix"], ir["nreact"], ir["nnz"], ir["nb"]
-    ndp = max(1, sum(len(s) for s in ir["dest"]))
     E = {}
     E["Y"] = 0
     E["YNEW"] = nvar
@@ -81,12 +80,12 @@ def layout(ir) -> dict:
     E["RCT"] = 9 * nvar
     E["FIX"] = E["RCT"] + nreact
     E["A"] = E["FIX"] + nfix
-    E["DP"] = E["A"] + nreact  # D_VAR term products
-    E["B"] = E["DP"] + ndp
+    E["B"] = E["A"] + nreact
     E["LU"] = E["B"] + nb
     E["ZERO"] = E["LU"] + nnz  # holds 0.0 (padding operand)
     E["SINK"] = E["ZERO"] + 1  # written by padding stores, never read
-    E["TOTAL"] = E["SINK"] + 1
+    E["ONE"] = E["SINK"] + 1   # holds 1.0 (missing factor of a D_VAR term)
+    E["TOTAL"] = E["ONE"] + 1
     # discrete adjoint: the forward block followed by the adjoint vectors
     E["LAM"] = E["TOTAL"]          # lambda = dL/dY at the end of the step
     E["LK"] = E["LAM"] + nvar      # 4 stage adjoints lambda_K
@@ -169,14 +168,6 @@ class Tables:
         return self.product_group([(f, vbase, self.off("A", r))
                                    for r, f in enumerate(self.ir["rate"])])
 
-    def dprods(self, vbase):
-        items, t = [], 0
-        for i, s in enumerate(self.ir["dest"]):
-            for sign, p in s:
-                items.append((p, vbase, self.off("DP", t)))
-                t += 1
-        return self.product_group(items)
-
     def bjac(self):
         return self.product_group([(f, "Y", self.off("B", b))
                                    for b, f in enumerate(self.ir["jac_b"]) if f is not None])
@@ -194,17 +185,27 @@ class Tables:
             coef = -coef
         return [resolve(x), 0, *_coef_words(coef)]
 
+    def d_term(self, sign, prod, vbase) -> list[int]:
+        """D_VAR term {f1, f2, f3, ci}: value ((c*f1)*f2)*f3 with c = lits[ci]
+        the signed literal (or +-1) and missing factors reading ONE.  Exact:
+        moving the literal in front is the product_shape rule, the sign of a
+        product is exact, x*1.0 == x."""
+        lit, data = self.product_shape(prod, vbase)
+        if len(data) > 3:
+            raise ValueError(f"D_VAR term with {len(data)} data factors: {prod}")
+        c = (lit if lit is not None else 1.0) * (-1.0 if sign < 0 else 1.0)
+        one = self.off("ONE", 0)
+        return [*data, *([one] * (3 - len(data))), self.lit_idx(c)]
+
     def pd(self, vbase) -> list[int]:
-        out, t = [], 0
+        out = []
         for i in range(self.ir["nvar"]):
             P, D = self.ir["prod"][i], self.ir["dest"][i]
             out += [len(P), len(D), self.off(vbase, i), i * 512]
             for s, p in P:
                 out += self.sum_term(s, p, lambda x: self.data_off(x, vbase))
             for s, p in D:
-                # the product itself is in DP[t]; the sign stays with the term
-                out += [self.off("DP", t), 0, *_coef_words(-1.0 if s < 0 else 1.0)]
-                t += 1
+                out += self.d_term(s, p, vbase)
         return out
 
     # -- Jacobian + LU ---------------------------------------------------------
@@ -446,7 +447,6 @@ def generate(ir: dict) -> str:
     body = []
     sizes = {}
     for name, (words, counts) in {"rate_y": t.rates("Y"), "rate_yn": t.rates("YNEW"),
-                                  "dprod_y": t.dprods("Y"), "dprod_yn": t.dprods("YNEW"),
                                   "bjac": t.bjac()}.items():
         body.append(_arr(f"gc_prod_{name}", words))
         body.append(_arr(f"gc_prodgrp_{name}", counts))

@@ -181,33 +181,69 @@ __device__ __forceinline__ double sum_terms(const Blk& b, ctab_t*& t, int n) {
   double acc = coef_of(t) * b.ld(t[0]);
   t += 4;
   int k = 1;
-  for (; k + 4 <= n; k += 4, t += 16) {
+  // eight loads in flight, then four, then one at a time
+  for (; k + 8 <= n; k += 8, t += 32) {
+    double x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = b.ld(t[4 * q]);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc = acc + coef_of(t + 4 * q) * x[q];
+  }
+  if (k + 4 <= n) {
     const double x0 = b.ld(t[0]), x1 = b.ld(t[4]), x2 = b.ld(t[8]), x3 = b.ld(t[12]);
     acc = acc + coef_of(t) * x0;
     acc = acc + coef_of(t + 4) * x1;
     acc = acc + coef_of(t + 8) * x2;
     acc = acc + coef_of(t + 12) * x3;
+    k += 4;
+    t += 16;
   }
   for (; k < n; ++k, t += 4) acc = acc + coef_of(t) * b.ld(t[0]);
   return acc;
 }
 
-// Fun_SPLIT (gckpp_Function.F90:2170-5060): rates, D_VAR term products, then
-// per species P_VAR, D_VAR and Vdot = P_VAR - D_VAR*V, stored at out + i*512.
-__device__ void fun(const Blk& b, bool ynew, int out) {
-  if (ynew) {
-    run_products(b, gc_prod_rate_yn, gc_prodgrp_rate_yn);
-    run_products(b, gc_prod_dprod_yn, gc_prodgrp_dprod_yn);
-  } else {
-    run_products(b, gc_prod_rate_y, gc_prodgrp_rate_y);
-    run_products(b, gc_prod_dprod_y, gc_prodgrp_dprod_y);
+// One D_VAR term {f1, f2, f3, ci}: ((lits[ci]*f1)*f2)*f3, the signed literal
+// first, missing factors reading ONE (1.0).
+__device__ __forceinline__ double dterm(const Blk& b, ctab_t* t) {
+  const double c = ((clit_t*)gc_lits)[t[3]];
+  return ((c * b.ld(t[0])) * b.ld(t[1])) * b.ld(t[2]);
+}
+
+// Sum of n >= 1 D_VAR terms in order (the first one sets the accumulator).
+__device__ __forceinline__ double sum_dterms(const Blk& b, ctab_t*& t, int n) {
+  double acc = dterm(b, t);
+  t += 4;
+  int k = 1;
+  for (; k + 4 <= n; k += 4, t += 16) {
+    double f[12];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      f[3 * q] = b.ld(t[4 * q]);
+      f[3 * q + 1] = b.ld(t[4 * q + 1]);
+      f[3 * q + 2] = b.ld(t[4 * q + 2]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double c = ((clit_t*)gc_lits)[t[4 * q + 3]];
+      acc = acc + ((c * f[3 * q]) * f[3 * q + 1]) * f[3 * q + 2];
+    }
   }
+  for (; k < n; ++k, t += 4) acc = acc + dterm(b, t);
+  return acc;
+}
+
+// Fun_SPLIT (gckpp_Function.F90:2170-5060): rates, then per species P_VAR,
+// D_VAR (term products evaluated in place) and Vdot = P_VAR - D_VAR*V,
+// stored at out + i*512.
+__device__ void fun(const Blk& b, bool ynew, int out) {
+  if (ynew) run_products(b, gc_prod_rate_yn, gc_prodgrp_rate_yn);
+  else run_products(b, gc_prod_rate_y, gc_prodgrp_rate_y);
   ctab_t* t = tab(ynew ? gc_pd_yn : gc_pd_y);
   for (int i = 0; i < NVAR; ++i) {
     const int nP = t[0], nD = t[1], vo = t[2], oo = t[3];
     t += 4;
     const double P = nP > 0 ? sum_terms(b, t, nP) : 0.0;
-    const double D = nD > 0 ? sum_terms(b, t, nD) : 0.0;
+    const double D = nD > 0 ? sum_dterms(b, t, nD) : 0.0;
     b.st(out + oo, P - D * b.ld(vo));
   }
 }
@@ -243,8 +279,19 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
       }
       const int n = op[3];
       op += 4;
-      // n is a multiple of 4; padding entries update the spare slot
-      for (int u = 0; u < n; u += 4, op += 8) {
+      // n is a multiple of 4; padding entries update the spare slot.  Eight
+      // updates per iteration while they last (distinct slots), then four.
+      int u = 0;
+      for (; u + 8 <= n; u += 8, op += 16) {
+        double uu[8], ww[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) uu[q] = b.ld(op[2 * q + 1]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) ww[q] = b.lds_ld(op[2 * q]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) b.lds_st(op[2 * q], ww[q] + a * uu[q]);
+      }
+      for (; u < n; u += 4, op += 8) {
         const double u0 = b.ld(op[1]), u1 = b.ld(op[3]), u2 = b.ld(op[5]), u3 = b.ld(op[7]);
         const double w0 = b.lds_ld(op[0]), w1 = b.lds_ld(op[2]);
         const double w2 = b.lds_ld(op[4]), w3 = b.lds_ld(op[6]);
@@ -274,7 +321,18 @@ __device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows) {
     const int n = t[0], xi = t[1], dg = t[2];
     t += 4;
     double x = b.ld(xi);
-    for (int u = 0; u < n; u += 4, t += 8) {
+    int u = 0;
+    for (; u + 8 <= n; u += 8, t += 16) {
+      double l[8], y[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        l[q] = b.ld(t[2 * q]);
+        y[q] = b.ld(t[2 * q + 1]);
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) x = x - l[q] * y[q];
+    }
+    for (; u < n; u += 4, t += 8) {
       const double l0 = b.ld(t[0]), x0 = b.ld(t[1]), l1 = b.ld(t[2]), x1 = b.ld(t[3]);
       const double l2 = b.ld(t[4]), x2 = b.ld(t[5]), l3 = b.ld(t[6]), x3 = b.ld(t[7]);
       x = x - l0 * x0;
@@ -338,6 +396,7 @@ __device__ void gather(const GcBatchArgs& a, long cell, const Blk& b) {
   for (int j = 0; j < NFIX; ++j) b.st(O_FIX + j * 512, c[NVAR + j]);
   for (int k = 0; k < NREACT; ++k) b.st(O_RCT + k * 512, r[k]);
   b.st(MECH_E_ZERO * 512, 0.0);  // padding operand of the solve tables
+  b.st(MECH_E_ONE * 512, 1.0);   // missing factor of a D_VAR term
 }
 
 __device__ void scatter(const GcBatchArgs& a, long cell, const Blk& b, const int* ist,
@@ -377,18 +436,23 @@ __device__ void stage_point(const Blk& b, int s) {
 __device__ void stages(const Blk& b, bool autonomous, double Dir, double H, int* ist) {
   for (int s = 0; s < 4; ++s) {
     const int Ks = O_K + s * O_VEC;
+    // K_s = Fcn + sum_j HC_j K_j, one pass (the adds in j order, as the
+    // reference's successive WAXPY calls)
+    int src = Ks;
     if (s == 0 || s == 1) {
       // stage 1: Fcn = Fcn0; stage 2 has ros_NewF = .FALSE. so Fcn is still Fcn0
-      for (int k = 0; k < NVAR; ++k) b.st(Ks + k * 512, b.ld(O_FCN0 + k * 512));
+      src = O_FCN0;
     } else {
       stage_point(b, s);
       fun(b, true, Ks);
       ist[0] += 1;
     }
-    for (int j = 0; j < s; ++j) {
-      const double HC = ros_C[s * (s - 1) / 2 + j] / (Dir * H);
-      for (int k = 0; k < NVAR; ++k)
-        b.st(Ks + k * 512, b.ld(Ks + k * 512) + HC * b.ld(O_K + j * O_VEC + k * 512));
+    double HC[3];
+    for (int j = 0; j < s; ++j) HC[j] = ros_C[s * (s - 1) / 2 + j] / (Dir * H);
+    for (int k = 0; k < NVAR; ++k) {
+      double v = b.ld(src + k * 512);
+      for (int j = 0; j < s; ++j) v = v + HC[j] * b.ld(O_K + j * O_VEC + k * 512);
+      b.st(Ks + k * 512, v);
     }
     if (!autonomous && ros_Gamma[s] != 0.0) {
       const double HG = (Dir * H) * ros_Gamma[s];
@@ -880,6 +944,7 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
   for (int j = 0; j < NFIX; ++j) b.st(O_FIX + j * 512, c[NVAR + j]);
   for (int r = 0; r < NREACT; ++r) b.st(O_RCT + r * 512, rc[r]);
   b.st(MECH_E_ZERO * 512, 0.0);
+  b.st(MECH_E_ONE * 512, 1.0);
   for (int i = 0; i < NVAR; ++i) b.st(O_LAM + i * 512, lam[i]);
   for (int j = 0; j < NFIX; ++j) b.st(O_GF + j * 512, lam[NVAR + j]);
   for (int r = 0; r < NREACT; ++r) b.st(O_GK + r * 512, 0.0);

@@ -40,7 +40,6 @@ class TableMachine:
         self.ir = ir
         self.seg = seg
         self.prod = {"rate_y": self.t.rates("Y"), "rate_yn": self.t.rates("YNEW"),
-                     "dprod_y": self.t.dprods("Y"), "dprod_yn": self.t.dprods("YNEW"),
                      "bjac": self.t.bjac()}
         self.pd = {"y": self.t.pd("Y"), "yn": self.t.pd("YNEW")}
         self.lu_ops = self.t.lu(seg)
@@ -52,6 +51,7 @@ class TableMachine:
         self.lits = np.array(self.t.lits or [0.0])
         self.ws = np.full((self.E["ADJ_TOTAL"], lanes), np.nan)
         self.ws[self.E["ZERO"]] = 0.0
+        self.ws[self.E["ONE"]] = 1.0
         self.lds = np.full((seg + 1, lanes), np.nan)
 
     # workspace access by byte offset e*512 (a lane's column)
@@ -85,10 +85,18 @@ class TableMachine:
             acc = acc + _coef(w[q + 2], w[q + 3]) * self.ld(w[q])
         return acc, p + 4 * n
 
+    def dterm(self, w, q):
+        return ((self.lits[w[q + 3]] * self.ld(w[q])) * self.ld(w[q + 1])) * self.ld(w[q + 2])
+
+    def sum_dterms(self, w, p, n):
+        acc = self.dterm(w, p)
+        for k in range(1, n):
+            acc = acc + self.dterm(w, p + 4 * k)
+        return acc, p + 4 * n
+
     def fun(self, ynew, out):
         sfx = "yn" if ynew else "y"
         self.products("rate_" + sfx)
-        self.products("dprod_" + sfx)
         w, p = self.pd[sfx], 0
         for _ in range(self.ir["nvar"]):
             nP, nD, vo, oo = w[p:p + 4]
@@ -97,7 +105,7 @@ class TableMachine:
             if nP:
                 P, p = self.sum_terms(w, p, nP)
             if nD:
-                D, p = self.sum_terms(w, p, nD)
+                D, p = self.sum_dterms(w, p, nD)
             self.st(out + oo, P - D * self.ld(vo))
         assert p == len(w)
 
@@ -191,16 +199,18 @@ class TableMachine:
         n = self.ir["nvar"]
         for s in range(4):
             Ks = self.o("K", s * n)
+            src = Ks
             if s < 2:
-                for k in range(n):
-                    self.st(Ks + k * 512, self.ld(self.o("FCN0", k)).copy())
+                src = self.o("FCN0")
             else:
                 self.stage_point(s)
                 self.fun(True, Ks)
-            for j in range(s):
-                hc = self.ROS_C[s * (s - 1) // 2 + j] / dh
-                for k in range(n):
-                    self.st(Ks + k * 512, self.ld(Ks + k * 512) + hc * self.ld(self.o("K", j * n + k)))
+            hc = [self.ROS_C[s * (s - 1) // 2 + j] / dh for j in range(s)]
+            for k in range(n):
+                v = self.ld(src + k * 512).copy()
+                for j in range(s):
+                    v = v + hc[j] * self.ld(self.o("K", j * n + k))
+                self.st(Ks + k * 512, v)
             self.solve(s)
 
     def adj_nu(self):
