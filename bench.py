@@ -138,6 +138,20 @@ def host_api_rate(lib, d, perm):
     return n / tT.value, tT.value, tK.value
 
 
+def membench(block_doubles, waves_per_cu, loads_per_store):
+    """Memory ceiling of the workspace access shape on this device
+    (csrc/gc_membench.hip, built by build.py): random 512-byte wave accesses
+    in per-wave blocks of the solver's size and read/write mix."""
+    import subprocess
+    exe = PKG / "lib" / "gc_membench"
+    if not exe.exists():
+        log("gc_membench is not built: no measured ceiling")
+        return None
+    out = subprocess.run([str(exe), str(block_doubles), str(waves_per_cu), str(loads_per_store)],
+                         check=True, capture_output=True, text=True, timeout=120).stdout
+    return json.loads(out.strip().splitlines()[-1])
+
+
 def copy_probe(torch, gib=1):
     """One elementwise copy kernel that reads and writes `gib` GiB: its
     FETCH_SIZE / WRITE_SIZE readings calibrate the counters of a profile."""
@@ -271,6 +285,13 @@ def main():
     log(f"persistent: {s_pers * 1e3:.1f} ms/step, kernel {tk_pers * 1e3:.1f} ms")
     s_plain, tk_plain = timed("plain", args.steps, 0)
     log(f"plain: {s_plain * 1e3:.1f} ms/step, kernel {tk_plain * 1e3:.1f} ms")
+    # the plain kernel in grid order (the order before any cost sort)
+    b.Perm = torch.arange(1, n + 1, dtype=torch.int32, device=b.C.device)
+    out = solver.solve(b, kernel="plain", out=out)
+    torch.cuda.synchronize()
+    tk_plain_grid = out.t_kernel
+    log(f"plain, grid order: kernel {tk_plain_grid * 1e3:.1f} ms")
+    b.Perm = torch.as_tensor(perm, device=b.C.device)
 
     host_rate, host_tt, host_tk = host_api_rate(lib, d, perm)
     adj = adjoint_ratio(mech) if not args.no_adjoint else None
@@ -289,6 +310,21 @@ def main():
         gbps = prof["bytes_per_launch"] / tk_pers / 1e9
         roof["hbm"] = {"achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                        "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+        # the ceiling this access shape reaches on this device, measured now
+        wpc = 4 * (2 if tune.get("variant", 1) == 1 else 4) if not tune.get("blocks_per_cu") \
+            else tune["blocks_per_cu"] * tune["block"] // 64
+        lps = max(1, round(prof["fetch_bytes"] / max(prof["write_bytes"], 1)))
+        mb = membench(L.load("fullchem").gc_amd_workspace_doubles_per_slot(), wpc, lps)
+        if mb:
+            ceil = mb["gather_tbs"] * 1e3
+            roof["hbm"].update({"pattern_ceiling": round(ceil, 1),
+                                "frac_of_pattern_ceiling": round(gbps / ceil, 4),
+                                "pattern": f"random 512-byte wave accesses in per-wave blocks "
+                                           f"of {mb['block_doubles']} doubles/lane, "
+                                           f"{mb['waves_per_cu']} waves/CU, 1 write per "
+                                           f"{mb['loads_per_store']} reads (gc_membench)",
+                                "stream_read": round(mb["stream_read_tbs"] * 1e3, 1)})
+            log(f"membench: {mb}")
 
     res = {
         "metric": METRIC,
@@ -308,6 +344,8 @@ def main():
                    "cells": n, "dt_s": 600.0, "kernel": "persistent", "tuning": tune},
         "plain_cells_per_s": round(n / s_plain, 1),
         "speedup_vs_plain": round(s_plain / s_pers, 3),
+        "plain_grid_order_cells_per_s": round(n / tk_plain_grid, 1),
+        "speedup_vs_plain_grid_order": round(tk_plain_grid / tk_pers, 3),
         "s_per_chem_step": round(s_pers, 4),
         "host_api_cells_per_s": round(host_rate, 1),
         "host_api_t_total_s": round(host_tt, 4),

@@ -138,10 +138,23 @@ def available_mechanisms():
     return sorted(p.stem for p in MECH_DIR.glob("*.json"))
 
 
+MEMBENCH = LIB / "gc_membench"
+
+
+def plan_membench(force: bool = False):
+    """The memory-ceiling calibration program (csrc/gc_membench.hip)."""
+    src = CSRC / "gc_membench.hip"
+    LIB.mkdir(exist_ok=True)
+    if force or _stale(MEMBENCH, [src]):
+        return [([HIPCC, *HIP_FLAGS, str(src), "-o", str(MEMBENCH)], "gc_membench")]
+    return []
+
+
 def build_all(force: bool = False, mechs=None) -> list[Path]:
-    """Build every mechanism library; all compile jobs share one pool."""
+    """Build every mechanism library and the calibration program; all compile
+    jobs share one pool."""
     plans = [plan_mechanism(m, force) for m in (mechs or available_mechanisms())]
-    todo = [job for jobs, _ in plans for job in jobs]
+    todo = [job for jobs, _ in plans for job in jobs] + plan_membench(force)
     if todo:
         _run_parallel(todo, max(1, min(len(todo), os.cpu_count() or 1)))
     return [link() for _, link in plans]
