@@ -1,11 +1,13 @@
 """Discrete adjoint of the batched Rodas3 solve, behind torch.autograd.Function.
 
 The forward pass is the persistent-kernel solve in its recording form: every
-accepted step's start state and step size go to a record pool in HBM (one
-record per accepted step, chained per cell).  The backward pass walks each
-cell's records from the last to the first on the GPU, recomputes the step's
-Jacobian, LU factors and stages from the recorded state, and sweeps the
-adjoint through the four stages in reverse with transposed solves on the
+accepted step's size goes to a record pool in HBM (one record per accepted
+step, chained per cell) and every `every`-th step's start state to a
+checkpoint pool.  The backward pass walks each cell's records from the last
+to the first on the GPU one segment at a time, replays the segment forward
+from its checkpoint to regain the start states, recomputes each step's
+Jacobian, LU factors and stages, and sweeps the adjoint through the four
+stages in reverse with transposed solves on the
 same factors (KppSolveTR order, gckpp_LinearAlgebra.F90:2327-3701), the
 stoichiometry-weighted adjoint of Fun_SPLIT, the rate-constant and
 fixed-species derivatives, and the second-order term of the Jacobian's
@@ -29,15 +31,19 @@ from . import lib as _lib
 from .solver import Batch, Result, _check
 
 DEFAULT_STEPS_PER_CELL = 128
+STATE_BUDGET_BYTES = 32e9   # checkpoint states kept in HBM before thinning them out
 
 
 @dataclass
 class Records:
-    Y: torch.Tensor      # [cap, NVAR]
-    H: torch.Tensor      # [cap]
-    prev: torch.Tensor   # [cap] int32
-    head: torch.Tensor   # [nCell] int32
+    H: torch.Tensor      # [cap] step size of each accepted step
+    prev: torch.Tensor   # [cap] int32, previous record of the cell (-1: first)
+    yslot: torch.Tensor  # [cap] int32, checkpoint state slot or -1
+    head: torch.Tensor   # [nCell] int32, last record of each cell
+    Y: torch.Tensor      # [ycap, NVAR] checkpoint states
+    every: int           # checkpoint interval
     used: int
+    yused: int
     t_kernel: float
 
 
@@ -45,39 +51,49 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-def record_solve(b: Batch, mech: str = "fullchem", cap: int = 0) -> tuple[Result, Records]:
-    """Forward solve of the batch (persistent kernel) keeping every accepted
-    step.  The pool is sized `cap` records (default 128 per cell) and grown
-    to the exact need if that was too small."""
+def record_solve(b: Batch, mech: str = "fullchem", every: int = 0, cap: int = 0,
+                 state_budget: float = STATE_BUDGET_BYTES) -> tuple[Result, Records]:
+    """Forward solve of the batch (persistent kernel) recording every accepted
+    step, with the start state of every `every`-th step of each cell as a
+    checkpoint.  every = 0 picks the smallest interval whose states fit
+    `state_budget` bytes; pools that turn out too small are re-sized to the
+    exact need and the solve is repeated."""
     lib = _lib.load(mech)
     nvar = _lib.mech_info(lib)[0]
     _check(b, _lib.mech_info(lib))
     n, dev = b.n, b.C.device
+    state_bytes = nvar * 8
     cap = int(cap or max(1, n * DEFAULT_STEPS_PER_CELL))
-    for _ in range(2):
+    ev = int(every) if every else max(1, -(-cap * state_bytes // int(state_budget)))
+    for _ in range(3):
+        ycap = max(1, min(cap, -(-cap // ev) + n))
         out = Result(b.C.clone(), torch.empty((n, 8), dtype=torch.int32, device=dev),
                      torch.empty((n, 3), dtype=torch.float64, device=dev),
                      torch.empty((n,), dtype=torch.int32, device=dev), 0.0)
-        recY = torch.empty((cap, nvar), dtype=torch.float64, device=dev)
         recH = torch.empty((cap,), dtype=torch.float64, device=dev)
         prev = torch.empty((cap,), dtype=torch.int32, device=dev)
+        yslot = torch.empty((cap,), dtype=torch.int32, device=dev)
         head = torch.empty((n,), dtype=torch.int32, device=dev)
-        used = ctypes.c_long(0)
+        recY = torch.empty((ycap, nvar), dtype=torch.float64, device=dev)
+        used, yused = ctypes.c_long(0), ctypes.c_long(0)
         tk = ctypes.c_double(0.0)
         stream = torch.cuda.current_stream(dev).cuda_stream
         rc = lib.gc_amd_record_batch_device(
             n, float(b.Tstart), float(b.Tend), _p(out.C), _p(b.RCONST), _p(b.ATOL), _p(b.RTOL),
             _p(b.ICNTRL), _p(b.RCNTRL), _p(b.Perm), _p(out.ISTAT), _p(out.RSTAT), _p(out.IERR),
-            _p(recY), _p(recH), _p(prev), _p(head), cap, ctypes.byref(used),
-            ctypes.c_void_p(stream), ctypes.byref(tk))
+            _p(recH), _p(prev), _p(yslot), _p(head), cap, _p(recY), ycap, ev,
+            ctypes.byref(used), ctypes.byref(yused), ctypes.c_void_p(stream), ctypes.byref(tk))
         if rc == -4:
-            del recY, recH, prev
-            cap = int(used.value)
+            del recH, prev, yslot, recY
+            cap = max(cap, int(used.value))
+            if not every:
+                ev = max(ev, -(-int(used.value) * state_bytes // int(state_budget)))
             continue
         if rc != 0:
             raise RuntimeError(f"gc_amd_record_batch_device returned {rc}: {_lib.last_error(lib)}")
         out.t_kernel = tk.value
-        return out, Records(recY, recH, prev, head, int(used.value), tk.value)
+        return out, Records(recH, prev, yslot, head, recY, ev, int(used.value), int(yused.value),
+                            tk.value)
     raise RuntimeError("record pool sizing failed")
 
 
@@ -98,8 +114,8 @@ def adjoint_sweep(b: Batch, rec: Records, lamC: torch.Tensor, mech: str = "fullc
     stream = torch.cuda.current_stream(dev).cuda_stream
     rc = lib.gc_amd_adjoint_batch_device(
         n, float(b.Tstart), float(b.Tend), _p(b.C), _p(b.RCONST), _p(b.ICNTRL), _p(perm),
-        _p(rec.Y), _p(rec.H), _p(rec.prev), _p(rec.head), int(rec.Y.shape[0]), _p(lamC),
-        _p(gC), _p(gR), _p(ierr),
+        _p(rec.H), _p(rec.prev), _p(rec.yslot), _p(rec.head), int(rec.H.shape[0]), _p(rec.Y),
+        int(rec.Y.shape[0]), int(rec.every), _p(lamC), _p(gC), _p(gR), _p(ierr),
         ctypes.c_void_p(stream), ctypes.byref(tk))
     if rc != 0:
         raise RuntimeError(f"gc_amd_adjoint_batch_device returned {rc}: {_lib.last_error(lib)}")

@@ -318,9 +318,10 @@ void fill_ierr(int* IERR, int n, int code) {
 
 // Recording forward solve (persistent kernel, one launch, no chunking).
 int record_solve(DevCtx& c, const GcBatchArgs& b, const GcRecord& rec_in, long* used,
-                 hipStream_t st, double* tKernel) {
+                 long* yused, hipStream_t st, double* tKernel) {
   if (tKernel) *tKernel = 0.0;
   if (used) *used = 0;
+  if (yused) *yused = 0;
   if (b.nCell == 0) return 0;
   const Tuning t = current_tuning();
   int rc = check_perm(c, b.Perm, b.nCell, st);
@@ -334,8 +335,9 @@ int record_solve(DevCtx& c, const GcBatchArgs& b, const GcRecord& rec_in, long* 
   if ((e = c.queue.ensure(256)) != hipSuccess) { set_error("hipMalloc(queue)", e); return -2; }
   unsigned* q = (unsigned*)c.queue.p;
   GcRecord rec = rec_in;
-  rec.count = q + 1;  // the second queue word counts records
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, q, 2);
+  rec.count = q + 1;   // the second queue word counts records,
+  rec.ycount = q + 2;  // the third one checkpoint states
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, q, 3);
   hipLaunchKernelGGL(k_fill_i32, dim3((b.nCell + 255) / 256), dim3(256), 0, st, rec.head,
                      b.nCell, -1);
   if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
@@ -348,17 +350,18 @@ int record_solve(DevCtx& c, const GcBatchArgs& b, const GcRecord& rec_in, long* 
           : gc_launch_record_v1(&b, (double*)c.ws.p, S, q, t.block, t.cells_per_wave, grain, &rec, st);
   if (e != hipSuccess) { set_error("launch", e); return -3; }
   if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
-  unsigned count = 0;
-  if ((e = hipMemcpyAsync(&count, rec.count, sizeof(unsigned), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+  unsigned counts[2] = {0, 0};
+  if ((e = hipMemcpyAsync(counts, rec.count, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st)) != hipSuccess ||
       (e = hipStreamSynchronize(st)) != hipSuccess) {
     set_error("record solve", e);
     return -3;
   }
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess && tKernel) *tKernel = ms * 1e-3;
-  if (used) *used = (long)count;
-  if (count > rec.cap) {
-    set_error("record pool too small: retry with cap >= *used");
+  if (used) *used = (long)counts[0];
+  if (yused) *yused = (long)counts[1];
+  if (counts[0] > rec.cap || counts[1] > rec.ycap) {
+    set_error("record pools too small: retry with cap >= *used and ycap >= *yused");
     return -4;
   }
   return 0;
@@ -372,14 +375,17 @@ int adjoint_solve(DevCtx& c, const GcAdjArgs& a, hipStream_t st, double* tKernel
   if (rc) return rc;
   hipError_t e;
   const size_t slots = ((size_t)a.nCell + 63u) & ~(size_t)63u;
-  const long per = variant_info(t.variant).adj_doubles_per_slot;
-  if ((e = c.ws.ensure(slots * (size_t)per * sizeof(double))) != hipSuccess) {
+  const GcVariantInfo vi = variant_info(t.variant);
+  GcAdjArgs aa = a;
+  // adjoint block + the segment buffers (every start states, step sizes)
+  aa.slot_doubles = vi.adj_doubles_per_slot + (long)a.every * (vi.nvar + 1);
+  if ((e = c.ws.ensure(slots * (size_t)aa.slot_doubles * sizeof(double))) != hipSuccess) {
     set_error("hipMalloc(workspace)", e);
     return -2;
   }
   if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
-  e = (t.variant == 2) ? gc_launch_adjoint_v2(&a, (double*)c.ws.p, 128, st)
-                       : gc_launch_adjoint_v1(&a, (double*)c.ws.p, 128, st);
+  e = (t.variant == 2) ? gc_launch_adjoint_v2(&aa, (double*)c.ws.p, 128, st)
+                       : gc_launch_adjoint_v1(&aa, (double*)c.ws.p, 128, st);
   if (e != hipSuccess) { set_error("launch", e); return -3; }
   if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
   if ((e = hipStreamSynchronize(st)) != hipSuccess) { set_error("adjoint", e); return -3; }
@@ -419,10 +425,12 @@ extern "C" {
 int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double* C,
                                const double* RCONST, const double* ATOL, const double* RTOL,
                                const int* ICNTRL, const double* RCNTRL, const int* Perm,
-                               int* ISTAT, double* RSTAT, int* IERR, double* recY,
-                               double* recH, int* recPrev, int* head, long cap, long* used,
-                               void* stream, double* tKernel) {
-  if (nCell < 0 || cap < 0 || cap > 0x7fffffffL) {
+                               int* ISTAT, double* RSTAT, int* IERR, double* recH,
+                               int* recPrev, int* recYslot, int* head, long cap, double* recY,
+                               long ycap, int every, long* used, long* yused, void* stream,
+                               double* tKernel) {
+  if (nCell < 0 || cap < 0 || cap > 0x7fffffffL || ycap < 0 || ycap > 0x7fffffffL ||
+      every < 1 || every > 4096) {
     set_error("gc_amd_record_batch_device: bad argument");
     return -1;
   }
@@ -430,31 +438,33 @@ int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double* C,
   DevCtx* cp = enter(stream, &rc);
   if (!cp) return rc;
   GcBatchArgs b{nCell, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR};
-  GcRecord rec{recY, recH, recPrev, head, nullptr, (unsigned)cap};
-  rc = record_solve(*cp, b, rec, used, cp->stream, tKernel);
+  GcRecord rec{recH, recPrev, recYslot, head, recY, nullptr, nullptr,
+               (unsigned)cap, (unsigned)ycap, every};
+  rc = record_solve(*cp, b, rec, used, yused, cp->stream, tKernel);
   cp->mu.unlock();
   return rc;
 }
 
 int gc_amd_adjoint_batch_device(int nCell, double Tstart, double Tend, const double* C,
                                 const double* RCONST, const int* ICNTRL, const int* Perm,
-                                const double* recY, const double* recH, const int* recPrev,
-                                const int* head, long cap, const double* lamC, double* gC,
-                                double* gRCONST, int* IERR, void* stream, double* tKernel) {
-  if (nCell < 0 || cap < 0 || cap > 0x7fffffffL) {
+                                const double* recH, const int* recPrev, const int* recYslot,
+                                const int* head, long cap, const double* recY, long ycap,
+                                int every, const double* lamC, double* gC, double* gRCONST,
+                                int* IERR, void* stream, double* tKernel) {
+  if (nCell < 0 || cap < 0 || cap > 0x7fffffffL || ycap < 0 || ycap > 0x7fffffffL ||
+      every < 1 || every > 64) {
     set_error("gc_amd_adjoint_batch_device: bad argument");
     return -1;
   }
   int rc = 0;
   DevCtx* cp = enter(stream, &rc);
   if (!cp) return rc;
-  GcAdjArgs a{nCell, Tstart, Tend, C, RCONST, ICNTRL, Perm, recY, recH, recPrev, head,
-              (unsigned)cap, lamC, gC, gRCONST, IERR};
+  GcAdjArgs a{nCell, Tstart, Tend, C, RCONST, ICNTRL, Perm, recY, recH, recPrev, recYslot,
+              head, (unsigned)cap, (unsigned)ycap, every, 0, lamC, gC, gRCONST, IERR};
   rc = adjoint_solve(*cp, a, cp->stream, tKernel);
   cp->mu.unlock();
   return rc;
 }
-
 
 int gc_gpu_set_device(int dev) {
   int n = 0;

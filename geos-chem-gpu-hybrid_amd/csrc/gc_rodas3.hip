@@ -580,24 +580,36 @@ GC_SFX(gc_rodas3_plain)(GcBatchArgs a, double* ws) {
 
 namespace {
 
-// Record of an accepted step (recording form): the state at its start and its
-// H go to a slot taken from the record pool (one atomic per wave), chained to
-// the cell's previous record.
-__device__ __forceinline__ void record_step(const Blk& b, const GcRecord& rec, bool acc,
-                                            double H, int lane, int& head) {
-  const unsigned long long mask = __ballot(acc);
-  if (mask == 0ull) return;
+// Wave-aggregated slot from a pool counter (one atomic per wave): the lanes
+// with `want` get consecutive slots; the others get ~0u.
+__device__ __forceinline__ unsigned take_slot(unsigned* counter, bool want, int lane) {
+  const unsigned long long mask = __ballot(want);
+  if (mask == 0ull) return ~0u;
   const int leader = __ffsll((long long)mask) - 1;
   unsigned base = 0;
-  if (lane == leader) base = atomicAdd(rec.count, (unsigned)__popcll(mask));
+  if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(mask));
   base = __shfl(base, leader);
-  if (!acc) return;
-  const unsigned slot = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-  if (slot >= rec.cap) return;  // pool exhausted: the host sees count > cap
-  double* y = rec.Y + (size_t)slot * NVAR;
-  for (int k = 0; k < NVAR; ++k) y[k] = b.ld(O_Y + k * 512);
+  return want ? base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull)) : ~0u;
+}
+
+// Record of an accepted step (recording form): its H and the cell's previous
+// record; the start state too when the step is a checkpoint (its index within
+// the cell, idx, a multiple of rec.every).  Called by every lane of the wave.
+__device__ __forceinline__ void record_step(const Blk& b, const GcRecord& rec, bool acc,
+                                            int idx, double H, int lane, int& head) {
+  const unsigned slot = take_slot(rec.count, acc, lane);
+  const bool ck = acc && (idx % rec.every) == 0;
+  const unsigned ys = take_slot(rec.ycount, ck, lane);
+  if (!acc || slot >= rec.cap) return;  // pool exhausted: the host sees count > cap
+  int yslot = -1;
+  if (ck && ys < rec.ycap) {
+    double* y = rec.Y + (size_t)ys * NVAR;
+    for (int k = 0; k < NVAR; ++k) y[k] = b.ld(O_Y + k * 512);
+    yslot = (int)ys;
+  }
   rec.H[slot] = H;
   rec.prev[slot] = head;
+  rec.yslot[slot] = yslot;
   head = (int)slot;
 }
 
@@ -730,7 +742,7 @@ __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws
       ist[2] += 1;
       accepted = (Err <= 1.0) || (H <= o.Hmin);
     }
-    if (REC) record_step(b, rec, accepted, H, lane, head);
+    if (REC) record_step(b, rec, accepted, ist[3], H, lane, head);
     if (factored) {
       if (accepted) {
         ist[3] += 1;
@@ -913,9 +925,11 @@ __device__ void adj_step(const Blk& b, double Dir, double H, bool clamp) {
 
 }  // namespace
 
-// One thread per cell in Perm order: walks the cell's recorded steps from the
-// last to the first, recomputing each step's factors and stages from its
-// start state, and sweeps the adjoint through it.
+// One thread per cell in Perm order: walks the cell's records from the last
+// to the first one segment at a time (the steps back to a checkpoint),
+// replays the segment forward from its checkpoint to regain each step's start
+// state, then recomputes each step's factors and stages and sweeps the adjoint
+// through it, last step first.
 extern "C" __global__ void
 __attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
 GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
@@ -923,7 +937,7 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
   const unsigned ic = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const Blk b = make_blk(ws, ic >> 6, lane, lds + (threadIdx.x >> 6) * ((GC_SEG + 1) * 64),
-                         MECH_E_ADJ_TOTAL);
+                         (int)a.slot_doubles);
   if (ic >= (unsigned)a.nCell) return;
   const long cell = (long)a.Perm[ic] - 1;
   const double* c = a.C + cell * NSPEC;
@@ -951,22 +965,69 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
   const double Dir = (a.Tend >= a.Tstart) ? 1.0 : -1.0;
   int ist[8];
   int ierr = 1;
+  // segment buffers after the adjoint block: start states, step sizes, records
+  const int m = a.every;
+  const int O_SEG = MECH_E_ADJ_TOTAL * 512;
+  const int O_SEGH = O_SEG + m * NVAR * 512;
   unsigned walked = 0;
-  for (int slot = a.head[cell]; slot >= 0; slot = a.recPrev[slot]) {
-    if ((unsigned)slot >= a.cap || ++walked > a.cap) {  // not a chain of this pool
+  int slot = a.head[cell];
+  while (slot >= 0 && ierr == 1) {
+    // one segment, walked backwards: records up to and including a checkpoint
+    int len = 0, ys = -1;
+    for (;;) {
+      if ((unsigned)slot >= a.cap || ++walked > a.cap || len >= m) {
+        ierr = -5;  // not a chain of these pools
+        break;
+      }
+      b.st(O_SEGH + len * 512, a.recH[slot]);
+      ++len;
+      ys = a.recYslot[slot];
+      slot = a.recPrev[slot];
+      if (ys >= 0) break;
+      if (slot < 0) {
+        ierr = -5;
+        break;
+      }
+    }
+    if (ierr != 1) break;
+    if ((unsigned)ys >= a.ycap) {
       ierr = -5;
       break;
     }
-    const double* y = a.recY + (size_t)slot * NVAR;
-    for (int i = 0; i < NVAR; ++i) b.st(O_Y + i * 512, y[i]);
-    const double H = a.recH[slot];
-    fun(b, false, O_FCN0);
-    if (jac_lu(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
-      ierr = -8;
-      break;
+    // replay forward from the checkpoint (bit-identical to the recorded run),
+    // keeping the start state of each of the segment's steps; SEGH holds the
+    // step sizes last step first
+    const double* y0 = a.recY + (size_t)ys * NVAR;
+    for (int i = 0; i < NVAR; ++i) b.st(O_Y + i * 512, y0[i]);
+    for (int q = 0; q < len; ++q) {
+      for (int i = 0; i < NVAR; ++i) b.st(O_SEG + (q * NVAR + i) * 512, b.ld(O_Y + i * 512));
+      if (q == len - 1) break;
+      const double H = b.ld(O_SEGH + (len - 1 - q) * 512);
+      fun(b, false, O_FCN0);
+      if (jac_lu(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
+        ierr = -8;
+        break;
+      }
+      stages(b, true, Dir, H, ist);
+      for (int k = 0; k < NVAR; ++k) {  // Ynew and the accepted update, as the stepper
+        double yn = b.ld(O_Y + k * 512);
+        for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * b.ld(O_K + j * O_VEC + k * 512);
+        b.st(O_Y + k * 512, clamp ? dmax(yn, 0.0) : yn);
+      }
     }
-    stages(b, true, Dir, H, ist);
-    adj_step(b, Dir, H, clamp);
+    if (ierr != 1) break;
+    // the adjoint through the segment's steps, last first
+    for (int q = len - 1; q >= 0; --q) {
+      const double H = b.ld(O_SEGH + (len - 1 - q) * 512);
+      for (int i = 0; i < NVAR; ++i) b.st(O_Y + i * 512, b.ld(O_SEG + (q * NVAR + i) * 512));
+      fun(b, false, O_FCN0);
+      if (jac_lu(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
+        ierr = -8;
+        break;
+      }
+      stages(b, true, Dir, H, ist);
+      adj_step(b, Dir, H, clamp);
+    }
   }
   for (int i = 0; i < NVAR; ++i) gc[i] = b.ld(O_LAM + i * 512);
   for (int j = 0; j < NFIX; ++j) gc[NVAR + j] = b.ld(O_GF + j * 512);

@@ -32,15 +32,21 @@ struct GcVariantInfo {
 };
 
 // Pool of accepted-step records kept by the recording form of the persistent
-// kernel for the discrete adjoint: record s holds the state Y(NVAR) at the
-// start of an accepted step, its H and the previous record of the same cell.
+// kernel for the discrete adjoint.  Every accepted step gets a record (its H
+// and the previous record of the same cell); every `every`-th accepted step
+// of a cell (its 1st, (every+1)-th, ...) is a checkpoint whose start state
+// Y(NVAR) also goes to the state pool.  The adjoint sweep replays the steps
+// between checkpoints.
 struct GcRecord {
-  double* Y;          // [cap][NVAR]
   double* H;          // [cap]
   int* prev;          // [cap], -1 for a cell's first step
+  int* yslot;         // [cap], state-pool slot of a checkpoint record, else -1
   int* head;          // [nCell] (caller's cell index): last record, -1 if none
+  double* Y;          // [ycap][NVAR]
   unsigned* count;    // records taken; > cap means the pool overflowed
-  unsigned cap;
+  unsigned* ycount;   // states taken; > ycap means the state pool overflowed
+  unsigned cap, ycap;
+  int every;          // checkpoint interval (>= 1)
 };
 
 // Arguments of the adjoint sweep (arrays in the caller's Fortran layout).
@@ -54,13 +60,17 @@ struct GcAdjArgs {
   const double* recY;
   const double* recH;
   const int* recPrev;
+  const int* recYslot;
   const int* head;
   unsigned cap;          // records in the pool (every slot followed is checked)
+  unsigned ycap;         // states in the state pool
+  int every;             // checkpoint interval of the recording
+  long slot_doubles;     // workspace per cell (forward + adjoint + segment)
   const double* lamC;    // (NSPEC, nCell): dL/dC at Tend
   double* gC;            // (NSPEC, nCell): dL/dC at Tstart
   double* gRCONST;       // (NREACT, nCell): dL/dRCONST
   int* IERR;             // (nCell): 1, -8 singular step, -3 non-autonomous,
-                         // -5 a record index outside the pool
+                         // -5 a record chain that does not fit the pools
 };
 
 // Internal to the library (not exported): the C-ABI in include/gcchemgpu.h is

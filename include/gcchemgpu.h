@@ -117,38 +117,46 @@ void gc_amd_get_tuning(int *block, int *blocks_per_cu, int *cells_per_wave,
 
 /* Forward batch solve (persistent kernel, same arguments and results as
  * gc_amd_integrate_batch_device with kernel = persistent and chunk = 0) that
- * also records every accepted step for the adjoint: record s holds the state
- * at the start of the step, recY[s*NVAR .. s*NVAR+NVAR), its step size recH[s]
- * and the previous record of the same cell recPrev[s] (-1 for the first);
+ * also records every accepted step for the adjoint.  Record s holds the
+ * step's size recH[s], the previous record of the same cell recPrev[s] (-1 for
+ * the first) and recYslot[s]: for every `every`-th accepted step of a cell
+ * (its 1st, (every+1)-th, ...) a checkpoint slot of the state pool, whose
+ * recY[slot*NVAR .. +NVAR) is the state at the start of the step, else -1.
  * head[c] (c = 0-based column of the caller's arrays) is the cell's last
- * record or -1.  The pool holds `cap` records; *used receives the number of
- * accepted steps.  Returns 0, -4 if *used > cap (the solve results are valid,
- * the records are not: call again with cap >= *used), or the codes of
- * gc_amd_integrate_batch_device.  Blocking; ordered after `stream`. */
+ * record or -1.  The pools hold `cap` records and `ycap` states; *used and
+ * *yused receive what the solve needed.  Returns 0, -4 if a pool was too small
+ * (the solve results are valid, the records are not: call again with pools of
+ * at least *used / *yused), or the codes of gc_amd_integrate_batch_device.
+ * Blocking; ordered after `stream`. */
 int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double *C,
                                const double *RCONST, const double *ATOL,
                                const double *RTOL, const int *ICNTRL,
                                const double *RCNTRL, const int *Perm, int *ISTAT,
-                               double *RSTAT, int *IERR, double *recY, double *recH,
-                               int *recPrev, int *head, long cap, long *used,
+                               double *RSTAT, int *IERR, double *recH, int *recPrev,
+                               int *recYslot, int *head, long cap, double *recY,
+                               long ycap, int every, long *used, long *yused,
                                void *stream, double *tKernel);
 
 /* Reverse sweep of the discrete adjoint of the recorded accepted Rodas3 steps
  * (step sizes held fixed).  Given lamC(NSPEC,nCell) = dL/dC at Tend, writes
  * gC(NSPEC,nCell) = dL/dC at Tstart (VAR and FIX) and gRCONST(NREACT,nCell) =
- * dL/dRCONST.  C supplies the fixed species; ICNTRL(1) must be non-zero
- * (autonomous, as GEOS-Chem sets it) and ICNTRL(16) = 1 (clamp) is honoured.
- * The record arrays are those of a successful gc_amd_record_batch_device
- * call with pool size `cap`.  IERR(c) = 1, -3 for a non-autonomous cell
- * (gradients 0), -8 if a recorded step's matrix is singular, -5 if the cell's
- * chain leaves the pool.  Returns 0 or a negative code as above.  Each
- * step's factors and stages are recomputed from its recorded start state;
- * the transposed solves reuse them (KppSolveTR order). */
+ * dL/dRCONST.  The record arguments are those of a successful
+ * gc_amd_record_batch_device call (same pools and `every`, 1..64).  Each
+ * segment of a cell's steps is replayed forward from its checkpoint (bit for
+ * bit the recorded trajectory), then each step's factors and stages are
+ * recomputed and the adjoint is swept through it with transposed solves on
+ * the same factors (KppSolveTR order).  C supplies the fixed species;
+ * ICNTRL(1) must be non-zero (autonomous, as GEOS-Chem sets it) and
+ * ICNTRL(16) = 1 (clamp) is honoured.  IERR(c) = 1, -3 for a non-autonomous
+ * cell (gradients 0), -8 if a recorded step's matrix is singular, -5 if the
+ * cell's chain does not fit the pools.  Returns 0 or a negative code as
+ * above.  Blocking; ordered after `stream`. */
 int gc_amd_adjoint_batch_device(int nCell, double Tstart, double Tend,
                                 const double *C, const double *RCONST,
                                 const int *ICNTRL, const int *Perm,
-                                const double *recY, const double *recH,
-                                const int *recPrev, const int *head, long cap,
+                                const double *recH, const int *recPrev,
+                                const int *recYslot, const int *head, long cap,
+                                const double *recY, long ycap, int every,
                                 const double *lamC, double *gC, double *gRCONST,
                                 int *IERR, void *stream, double *tKernel);
 
