@@ -216,6 +216,82 @@ def bench_grid_2x25(args, mech, lib, tune):
     }
 
 
+STRESS_SIZES = (1000, 4000, 16000, 32000, 64000, 100000, 150000, 200000)
+
+
+def bench_stress(args):
+    """Synthetic stress mechanism (500 species, 1,500 reactions, ~3% LU fill)
+    in eight ragged batches of 1k-200k cells, dt 600 s, launch order = batch
+    order (no previous step to sort by): each batch with the persistent and
+    the plain kernel, then all batches in one persistent launch when the
+    per-batch timings say it finishes in under 100 s, and the adjoint
+    (record + sweep, loss gradient dL/dC_final = 1) of the first batch."""
+    import numpy as np
+    import torch
+    from gcchem_amd import adjoint, inputs, mechanism, solver
+    mech = mechanism.load("stress500")
+    sizes = [int(x) for x in args.stress_sizes.split(",")] if args.stress_sizes else STRESS_SIZES
+    per, total_p, total_q, adj = [], 0.0, 0.0, []
+    batches = []
+    for i, n in enumerate(sizes):
+        d = inputs.config4(mech, n, seed=40 + i)
+        b = solver.Batch.from_numpy(d)
+        rp = solver.solve(b, kernel="persistent", mech="stress500")
+        torch.cuda.synchronize()
+        ok = float((rp.IERR == 1).float().mean().item())
+        nst = float(rp.ISTAT[:, 2].float().mean())
+        log(f"stress batch {n}: persistent {rp.t_kernel:.2f} s, mean steps {nst:.0f}, "
+            f"IERR=1 fraction {ok:.4f} (the rest -6, for the caller's CPU retry)")
+        rq = solver.solve(b, kernel="plain", mech="stress500")
+        torch.cuda.synchronize()
+        log(f"stress batch {n}: plain {rq.t_kernel:.2f} s")
+        per.append({"cells": n, "persistent_s": round(rp.t_kernel, 3), "plain_s": round(rq.t_kernel, 3),
+                    "mean_steps": round(nst, 1), "ierr_ok_frac": ok})
+        total_p += rp.t_kernel
+        total_q += rq.t_kernel
+        if i == 0 and not args.no_adjoint:
+            out, rec = adjoint.record_solve(b, mech="stress500")
+            torch.cuda.synchronize()
+            log(f"stress batch {n}: recording forward {rec.t_kernel:.2f} s")
+            lam = torch.ones_like(out.C)
+            g, gr, ierr, t_sweep = adjoint.adjoint_sweep(b, rec, lam, mech="stress500")
+            torch.cuda.synchronize()
+            ratio = (rec.t_kernel + t_sweep) / rp.t_kernel
+            log(f"stress batch {n}: adjoint record {rec.t_kernel:.2f} s + sweep {t_sweep:.2f} s "
+                f"(checkpoint every {rec.every}), adj/fwd {ratio:.2f}")
+            adj.append({"cells": n, "record_s": round(rec.t_kernel, 3), "sweep_s": round(t_sweep, 3),
+                        "every": rec.every, "adj_over_fwd": round(ratio, 3),
+                        "ierr_ok_frac": float((ierr == 1).float().mean().item())})
+        batches.append(d)
+        del b
+    ncell = sum(sizes)
+    one = None
+    if total_p < 100.0:
+        cat = {k: np.concatenate([d[k] for d in batches]) for k in ("C", "RCONST", "ICNTRL", "RCNTRL")}
+        cat.update({k: batches[0][k] for k in ("ATOL", "RTOL", "Tstart", "Tend")})
+        cat["Perm"] = np.arange(1, ncell + 1, dtype=np.int32)
+        b = solver.Batch.from_numpy(cat)
+        r = solver.solve(b, kernel="persistent", mech="stress500")
+        torch.cuda.synchronize()
+        one = round(r.t_kernel, 3)
+        log(f"all {len(sizes)} batches in one persistent launch: {r.t_kernel:.2f} s")
+    t = one if one is not None else total_p
+    return {
+        "metric": METRIC, "value": round(ncell / t, 1), "unit": "cells/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(t * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"synthetic stress mechanism 500 species x 1500 reactions, "
+                               f"{len(sizes)} ragged batches {sizes}, dt 600 s, "
+                               f"Max_no_steps 1000 (longer cells -> IERR -6, CPU retry)",
+                   "cells": ncell, "one_launch": one is not None},
+        "plain_cells_per_s": round(ncell / total_q, 1),
+        "speedup_vs_plain": round(total_q / t, 3),
+        "batches": per, "adjoint": adj,
+        "adj_over_fwd": adj[-1]["adj_over_fwd"] if adj else None,
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,9 +300,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells", type=int, default=0, help="use the first N cells (debug)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
-    ap.add_argument("--grid", default="4x5", choices=["4x5", "2x2.5"],
-                    help="4x5 (238,464 cells; the headline line) or 2x2.5 (943,488 cells, "
-                         "chunked graph)")
+    ap.add_argument("--grid", default="4x5", choices=["4x5", "2x2.5", "stress"],
+                    help="4x5 (238,464 cells; the headline line), 2x2.5 (943,488 cells, "
+                         "chunked graph) or stress (synthetic 500x1500 mechanism, ragged batches)")
+    ap.add_argument("--stress-sizes", default="", help="comma-separated batch sizes (stress)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB elementwise copy first (counter calibration)")
     args = ap.parse_args()
@@ -248,6 +325,9 @@ def main():
     tune = tuning.apply(lib, "fullchem")
     if args.grid == "2x2.5":
         print(json.dumps(bench_grid_2x25(args, mech, lib, tune)), flush=True)
+        return
+    if args.grid == "stress":
+        print(json.dumps(bench_stress(args)), flush=True)
         return
     t0 = time.time()
     d = inputs.config1(mech, seed=1)

@@ -217,11 +217,15 @@ def cost_order(nsteps, descending: bool = True):
     return (order + 1).astype(np.int32)
 
 
-def config4(mech: Mechanism, n: int, seed: int = 4) -> dict:
+def config4(mech: Mechanism, n: int, seed: int = 4, max_steps: int = 1000) -> dict:
     """Cells of a synthetic stress mechanism: concentrations log-uniform
     [1e-2, 1e10]; rate constants log-uniform [1e-6, 1e4] s^-1 in scaled units,
     i.e. RCONST = k / (1e6)^(order-1) so that k is the pseudo-first-order rate
-    at a concentration of 1e6; dt = 600 s; fixed species 1e10."""
+    at a concentration of 1e6; dt = 600 s; fixed species 1e10.  The random
+    initial states give a stiff transient (median ~400 accepted steps, a few
+    cells in a thousand need 10^4), so Max_no_steps = ICNTRL(4) is set to
+    `max_steps`: cells beyond it end with IERR = -6 for the caller's CPU retry,
+    as GEOS-Chem handles failed cells."""
     rng = np.random.default_rng(seed)
     C = np.empty((n, mech.nspec))
     C[:, :mech.nvar] = _logu(rng, 1e-2, 1e10, (n, mech.nvar))
@@ -230,5 +234,6 @@ def config4(mech: Mechanism, n: int, seed: int = 4) -> dict:
     k = _logu(rng, 1e-6, 1e4, (n, mech.nreact))
     RCONST = k / (1e6 ** np.maximum(order - 1, 0))[None, :]
     ICNTRL, RCNTRL, ATOL, RTOL = _controls(n, mech.nvar)
+    ICNTRL[:, 3] = max_steps
     return dict(C=C, RCONST=RCONST, ICNTRL=ICNTRL, RCNTRL=RCNTRL, ATOL=ATOL, RTOL=RTOL,
                 Perm=np.arange(1, n + 1, dtype=np.int32), Tstart=0.0, Tend=600.0)
