@@ -424,8 +424,15 @@ class Tables:
 
 
 def _arr(name, vals, ctype="int") -> str:
+    """A table as a 64-byte aligned device array.  Int tables are padded with
+    zeros to a multiple of 16 words plus 16, so the kernels may fetch 16-byte
+    groups of words (every record starts on a 4-word boundary) and read one
+    group past a record without leaving the array."""
+    vals = list(vals)
+    if ctype == "int":
+        vals += [0] * ((-len(vals)) % 16 + 16)
     body = ",".join(str(v) for v in vals) if vals else "0"
-    return f"__device__ const {ctype} {name}[] = {{{body}}};\n"
+    return f"__device__ const {ctype} {name}[] __attribute__((aligned(64))) = {{{body}}};\n"
 
 
 def generate(ir: dict) -> str:

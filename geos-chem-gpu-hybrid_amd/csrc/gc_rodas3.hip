@@ -127,6 +127,19 @@ __device__ __forceinline__ Blk make_blk(double* ws, unsigned wave, int lane, dou
   return b;
 }
 
+// Table words arrive in 16-byte groups (one s_load_dwordx4 each): every
+// record starts on a 4-word boundary of a 64-byte aligned table, and every
+// table has 16 words of zero padding after its last record.
+typedef int i4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(4))) const i4 ci4_t;
+
+__device__ __forceinline__ i4 ld4(ctab_t* p) { return *(ci4_t*)p; }
+
+__device__ __forceinline__ double dbl(int lo, int hi) {
+  return __builtin_bit_cast(double, (unsigned long long)(unsigned)lo |
+                                        ((unsigned long long)(unsigned)hi << 32));
+}
+
 __device__ __forceinline__ double coef_of(ctab_t* t) {
   const unsigned long long lo = (unsigned)t[2], hi = (unsigned)t[3];
   return __builtin_bit_cast(double, lo | (hi << 32));
@@ -134,13 +147,14 @@ __device__ __forceinline__ double coef_of(ctab_t* t) {
 
 // One product record {dst, lit, f0, f1, f2, f3, 0, 0} of shape (NF, LIT):
 // lit*f0*f1*... left to right.
+// Record words a = {dst, lit, f0, f1}, c = {f2, f3, 0, 0}.
 template <int NF, bool LIT>
-__device__ __forceinline__ double prod_rec(const Blk& b, ctab_t* rec) {
-  double v = b.ld(rec[2]);
-  if (LIT) v = ((clit_t*)gc_lits)[rec[1]] * v;
-  if (NF > 1) v = v * b.ld(rec[3]);
-  if (NF > 2) v = v * b.ld(rec[4]);
-  if (NF > 3) v = v * b.ld(rec[5]);
+__device__ __forceinline__ double prod_rec(const Blk& b, i4 a, i4 c) {
+  double v = b.ld(a.z);
+  if (LIT) v = ((clit_t*)gc_lits)[a.y] * v;
+  if (NF > 1) v = v * b.ld(a.w);
+  if (NF > 2) v = v * b.ld(c.x);
+  if (NF > 3) v = v * b.ld(c.y);
   return v;
 }
 
@@ -148,13 +162,15 @@ template <int NF, bool LIT>
 __device__ __forceinline__ ctab_t* prod_group(const Blk& b, ctab_t* rec, int n) {
   int r = 0;
   for (; r + 2 <= n; r += 2, rec += 16) {
-    const double v0 = prod_rec<NF, LIT>(b, rec);
-    const double v1 = prod_rec<NF, LIT>(b, rec + 8);
-    b.st(rec[0], v0);
-    b.st(rec[8], v1);
+    const i4 a0 = ld4(rec), c0 = ld4(rec + 4), a1 = ld4(rec + 8), c1 = ld4(rec + 12);
+    const double v0 = prod_rec<NF, LIT>(b, a0, c0);
+    const double v1 = prod_rec<NF, LIT>(b, a1, c1);
+    b.st(a0.x, v0);
+    b.st(a1.x, v1);
   }
   if (r < n) {
-    b.st(rec[0], prod_rec<NF, LIT>(b, rec));
+    const i4 a0 = ld4(rec), c0 = ld4(rec + 4);
+    b.st(a0.x, prod_rec<NF, LIT>(b, a0, c0));
     rec += 8;
   }
   return rec;
@@ -176,59 +192,57 @@ __device__ void run_products(const Blk& b, const int* table, const int* grp_) {
 }
 
 // Sum of n >= 1 terms {soff, 0, coef}: acc = c0*x0, then acc = acc + c*x in
-// order (signs are folded into the coefficients).  Advances t past the terms.
-__device__ __forceinline__ double sum_terms(const Blk& b, ctab_t*& t, int n) {
-  double acc = coef_of(t) * b.ld(t[0]);
-  t += 4;
+// order (signs are folded into the coefficients).  The first term's words
+// come in `first` (fetched with the record header); t points at the second
+// term and is advanced past the last one.
+__device__ __forceinline__ double sum_terms(const Blk& b, i4 first, ctab_t*& t, int n) {
+  double acc = dbl(first.z, first.w) * b.ld(first.x);
   int k = 1;
-  // eight loads in flight, then four, then one at a time
-  for (; k + 8 <= n; k += 8, t += 32) {
-    double x[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) x[q] = b.ld(t[4 * q]);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) acc = acc + coef_of(t + 4 * q) * x[q];
+  for (; k + 4 <= n; k += 4, t += 16) {
+    const i4 q0 = ld4(t), q1 = ld4(t + 4), q2 = ld4(t + 8), q3 = ld4(t + 12);
+    const double x0 = b.ld(q0.x), x1 = b.ld(q1.x), x2 = b.ld(q2.x), x3 = b.ld(q3.x);
+    acc = acc + dbl(q0.z, q0.w) * x0;
+    acc = acc + dbl(q1.z, q1.w) * x1;
+    acc = acc + dbl(q2.z, q2.w) * x2;
+    acc = acc + dbl(q3.z, q3.w) * x3;
   }
-  if (k + 4 <= n) {
-    const double x0 = b.ld(t[0]), x1 = b.ld(t[4]), x2 = b.ld(t[8]), x3 = b.ld(t[12]);
-    acc = acc + coef_of(t) * x0;
-    acc = acc + coef_of(t + 4) * x1;
-    acc = acc + coef_of(t + 8) * x2;
-    acc = acc + coef_of(t + 12) * x3;
-    k += 4;
-    t += 16;
+  for (; k < n; ++k, t += 4) {
+    const i4 q = ld4(t);
+    acc = acc + dbl(q.z, q.w) * b.ld(q.x);
   }
-  for (; k < n; ++k, t += 4) acc = acc + coef_of(t) * b.ld(t[0]);
   return acc;
+}
+
+__device__ __forceinline__ double sum_terms(const Blk& b, ctab_t*& t, int n) {
+  const i4 first = ld4(t);
+  t += 4;
+  return sum_terms(b, first, t, n);
 }
 
 // One D_VAR term {f1, f2, f3, ci}: ((lits[ci]*f1)*f2)*f3, the signed literal
 // first, missing factors reading ONE (1.0).
-__device__ __forceinline__ double dterm(const Blk& b, ctab_t* t) {
-  const double c = ((clit_t*)gc_lits)[t[3]];
-  return ((c * b.ld(t[0])) * b.ld(t[1])) * b.ld(t[2]);
+__device__ __forceinline__ double dterm(const Blk& b, i4 q) {
+  const double c = ((clit_t*)gc_lits)[q.w];
+  return ((c * b.ld(q.x)) * b.ld(q.y)) * b.ld(q.z);
 }
 
 // Sum of n >= 1 D_VAR terms in order (the first one sets the accumulator).
 __device__ __forceinline__ double sum_dterms(const Blk& b, ctab_t*& t, int n) {
-  double acc = dterm(b, t);
+  double acc = dterm(b, ld4(t));
   t += 4;
   int k = 1;
   for (; k + 4 <= n; k += 4, t += 16) {
-    double f[12];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      f[3 * q] = b.ld(t[4 * q]);
-      f[3 * q + 1] = b.ld(t[4 * q + 1]);
-      f[3 * q + 2] = b.ld(t[4 * q + 2]);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double c = ((clit_t*)gc_lits)[t[4 * q + 3]];
-      acc = acc + ((c * f[3 * q]) * f[3 * q + 1]) * f[3 * q + 2];
-    }
+    const i4 q0 = ld4(t), q1 = ld4(t + 4), q2 = ld4(t + 8), q3 = ld4(t + 12);
+    const double f00 = b.ld(q0.x), f01 = b.ld(q0.y), f02 = b.ld(q0.z);
+    const double f10 = b.ld(q1.x), f11 = b.ld(q1.y), f12 = b.ld(q1.z);
+    const double f20 = b.ld(q2.x), f21 = b.ld(q2.y), f22 = b.ld(q2.z);
+    const double f30 = b.ld(q3.x), f31 = b.ld(q3.y), f32 = b.ld(q3.z);
+    acc = acc + ((((clit_t*)gc_lits)[q0.w] * f00) * f01) * f02;
+    acc = acc + ((((clit_t*)gc_lits)[q1.w] * f10) * f11) * f12;
+    acc = acc + ((((clit_t*)gc_lits)[q2.w] * f20) * f21) * f22;
+    acc = acc + ((((clit_t*)gc_lits)[q3.w] * f30) * f31) * f32;
   }
-  for (; k < n; ++k, t += 4) acc = acc + dterm(b, t);
+  for (; k < n; ++k, t += 4) acc = acc + dterm(b, ld4(t));
   return acc;
 }
 
@@ -240,9 +254,15 @@ __device__ void fun(const Blk& b, bool ynew, int out) {
   else run_products(b, gc_prod_rate_y, gc_prodgrp_rate_y);
   ctab_t* t = tab(ynew ? gc_pd_yn : gc_pd_y);
   for (int i = 0; i < NVAR; ++i) {
-    const int nP = t[0], nD = t[1], vo = t[2], oo = t[3];
-    t += 4;
-    const double P = nP > 0 ? sum_terms(b, t, nP) : 0.0;
+    const i4 h = ld4(t), f = ld4(t + 4);  // header and the first term
+    const int nP = h.x, nD = h.y, vo = h.z, oo = h.w;
+    t += 8;
+    double P = 0.0;
+    if (nP > 0) {
+      P = sum_terms(b, f, t, nP);
+    } else {
+      t -= 4;
+    }
     const double D = nD > 0 ? sum_dterms(b, t, nD) : 0.0;
     b.st(out + oo, P - D * b.ld(vo));
   }
@@ -256,12 +276,18 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
   ctab_t* op = tab(GC_LU_TABLE);
   int sing = 0;
   for (;;) {
-    const int code = op[0];
+    const i4 h = ld4(op), f = ld4(op + 4);  // header and the first 4 words after it
+    const int code = h.x;
     if (code == 0) break;
     if (code == 1) {
-      const int slot = op[1], nt = op[2], isdiag = op[3];
-      op += 4;
-      const double v = nt > 0 ? sum_terms(b, op, nt) : 0.0;
+      const int slot = h.y, nt = h.z, isdiag = h.w;
+      op += 8;
+      double v = 0.0;
+      if (nt > 0) {
+        v = sum_terms(b, f, op, nt);
+      } else {
+        op -= 4;
+      }
       double g = -v;
       if (isdiag) {
         g = g + ghinv;
@@ -269,44 +295,52 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
       }
       b.lds_st(slot, g);
     } else if (code == 2 || code == 3) {
+      // n updates {slot, U} (n a multiple of 4; padding entries update the
+      // spare slot), four per iteration; the first four are loaded together
+      // with the pivot operand
+      const int n = h.w;
+      op += 4;
+      i4 p01 = f, p23 = ld4(op + 4);
+      double u0 = 0.0, u1 = 0.0, u2 = 0.0, u3 = 0.0;
+      if (n > 0) {
+        u0 = b.ld(p01.y);
+        u1 = b.ld(p01.w);
+        u2 = b.ld(p23.y);
+        u3 = b.ld(p23.w);
+      }
       double a;
       if (code == 2) {
-        const int sj = op[1];
-        a = -b.lds_ld(sj) / b.ld(op[2]);
+        const int sj = h.y;
+        a = -b.lds_ld(sj) / b.ld(h.z);
         b.lds_st(sj, -a);
       } else {
-        a = -b.ld(op[1]);
+        a = -b.ld(h.y);
       }
-      const int n = op[3];
+      for (int u = 0; u < n; u += 4, op += 8) {
+        if (u > 0) {
+          p01 = ld4(op);
+          p23 = ld4(op + 4);
+          u0 = b.ld(p01.y);
+          u1 = b.ld(p01.w);
+          u2 = b.ld(p23.y);
+          u3 = b.ld(p23.w);
+        }
+        const double w0 = b.lds_ld(p01.x), w1 = b.lds_ld(p01.z);
+        const double w2 = b.lds_ld(p23.x), w3 = b.lds_ld(p23.z);
+        b.lds_st(p01.x, w0 + a * u0);
+        b.lds_st(p01.z, w1 + a * u1);
+        b.lds_st(p23.x, w2 + a * u2);
+        b.lds_st(p23.z, w3 + a * u3);
+      }
+    } else {  // code 4: store the segment, pairs {slot, LU} (padding writes the sink)
+      const int n = h.y;
       op += 4;
-      // n is a multiple of 4; padding entries update the spare slot.  Eight
-      // updates per iteration while they last (distinct slots), then four.
-      int u = 0;
-      for (; u + 8 <= n; u += 8, op += 16) {
-        double uu[8], ww[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) uu[q] = b.ld(op[2 * q + 1]);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) ww[q] = b.lds_ld(op[2 * q]);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) b.lds_st(op[2 * q], ww[q] + a * uu[q]);
-      }
-      for (; u < n; u += 4, op += 8) {
-        const double u0 = b.ld(op[1]), u1 = b.ld(op[3]), u2 = b.ld(op[5]), u3 = b.ld(op[7]);
-        const double w0 = b.lds_ld(op[0]), w1 = b.lds_ld(op[2]);
-        const double w2 = b.lds_ld(op[4]), w3 = b.lds_ld(op[6]);
-        b.lds_st(op[0], w0 + a * u0);
-        b.lds_st(op[2], w1 + a * u1);
-        b.lds_st(op[4], w2 + a * u2);
-        b.lds_st(op[6], w3 + a * u3);
-      }
-    } else {  // code 4: store the segment (pairs; padding writes the sink)
-      const int n = op[1];
-      op += 4;
+      i4 q = f;
       for (int u = 0; u < n; u += 2, op += 4) {
-        const double v0 = b.lds_ld(op[0]), v1 = b.lds_ld(op[2]);
-        b.st(op[1], v0);
-        b.st(op[3], v1);
+        if (u > 0) q = ld4(op);
+        const double v0 = b.lds_ld(q.x), v1 = b.lds_ld(q.z);
+        b.st(q.y, v0);
+        b.st(q.w, v1);
       }
     }
   }
@@ -318,23 +352,23 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
 template <bool DIV>
 __device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows) {
   for (int r = 0; r < rows; ++r) {
-    const int n = t[0], xi = t[1], dg = t[2];
+    const i4 h = ld4(t), p01 = ld4(t + 4), p23 = ld4(t + 8);  // header, first four pairs
+    const int n = h.x, xi = h.y, dg = h.z;
     t += 4;
     double x = b.ld(xi);
-    int u = 0;
-    for (; u + 8 <= n; u += 8, t += 16) {
-      double l[8], y[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        l[q] = b.ld(t[2 * q]);
-        y[q] = b.ld(t[2 * q + 1]);
-      }
-#pragma unroll
-      for (int q = 0; q < 8; ++q) x = x - l[q] * y[q];
+    if (n > 0) {
+      const double l0 = b.ld(p01.x), x0 = b.ld(p01.y), l1 = b.ld(p01.z), x1 = b.ld(p01.w);
+      const double l2 = b.ld(p23.x), x2 = b.ld(p23.y), l3 = b.ld(p23.z), x3 = b.ld(p23.w);
+      x = x - l0 * x0;
+      x = x - l1 * x1;
+      x = x - l2 * x2;
+      x = x - l3 * x3;
+      t += 8;
     }
-    for (; u < n; u += 4, t += 8) {
-      const double l0 = b.ld(t[0]), x0 = b.ld(t[1]), l1 = b.ld(t[2]), x1 = b.ld(t[3]);
-      const double l2 = b.ld(t[4]), x2 = b.ld(t[5]), l3 = b.ld(t[6]), x3 = b.ld(t[7]);
+    for (int u = 4; u < n; u += 4, t += 8) {
+      const i4 q01 = ld4(t), q23 = ld4(t + 4);
+      const double l0 = b.ld(q01.x), x0 = b.ld(q01.y), l1 = b.ld(q01.z), x1 = b.ld(q01.w);
+      const double l2 = b.ld(q23.x), x2 = b.ld(q23.y), l3 = b.ld(q23.z), x3 = b.ld(q23.w);
       x = x - l0 * x0;
       x = x - l1 * x1;
       x = x - l2 * x2;
