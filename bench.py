@@ -391,7 +391,7 @@ def main():
         roof["hbm"] = {"achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                        "frac": round(gbps / HBM_PEAK_GBPS, 4)}
         # the ceiling this access shape reaches on this device, measured now
-        wpc = 4 * (2 if tune.get("variant", 1) == 1 else 4) if not tune.get("blocks_per_cu") \
+        wpc = 4 * {1: 2, 2: 4, 3: 3}[tune.get("variant", 1)] if not tune.get("blocks_per_cu") \
             else tune["blocks_per_cu"] * tune["block"] // 64
         lps = max(1, round(prof["fetch_bytes"] / max(prof["write_bytes"], 1)))
         mb = membench(L.load("fullchem").gc_amd_workspace_doubles_per_slot(), wpc, lps)

@@ -2,7 +2,8 @@
 
 Coordinate descent over the five launch parameters of gc_amd_set_tuning:
 
-* ``variant``         LU row-segment size in LDS (1: 32 entries, 2: 16 entries)
+* ``variant``         LU row-segment size in LDS and occupancy target
+                      (1: 32 entries, 8 waves/CU; 2: 16, 16; 3: 24, 12)
 * ``block``           work-group size (64-256 lanes)
 * ``blocks_per_cu``   resident work-groups per compute unit (0: fill the CU)
 * ``cells_per_wave``  lanes of each wave that take cells (16-64)
@@ -33,15 +34,15 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 
 SPACE = {
-    "variant": [1, 2],
+    "variant": [1, 2, 3],
     "block": [64, 128, 256],
     "blocks_per_cu": [0, 1, 2, 3, 4, 6, 8],
     "cells_per_wave": [16, 32, 48, 64],
     "grain": [1, 16, 64, 256, 1024, 4096],
 }
 LDS_PER_CU = 160 * 1024
-SEG = {1: 32, 2: 16}
-WAVES_PER_SIMD = {1: 2, 2: 4}
+SEG = {1: 32, 2: 16, 3: 24}
+WAVES_PER_SIMD = {1: 2, 2: 4, 3: 3}
 
 
 def _pkg():

@@ -39,7 +39,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-VARIANTS = (1, 2)  # kernel variants (GC_VARIANT)
+VARIANTS = (1, 2, 3)  # kernel variants (GC_VARIANT)
 
 
 def lib_path(mech: str = "fullchem") -> Path:

@@ -65,7 +65,7 @@ from fractions import Fraction
 import sys
 from pathlib import Path
 
-SEGMENTS = (16, 32)     # LU row-segment sizes compiled (LDS entries per lane)
+SEGMENTS = (16, 24, 32)  # LU row-segment sizes compiled (LDS entries per lane)
 
 
 def layout(ir) -> dict:

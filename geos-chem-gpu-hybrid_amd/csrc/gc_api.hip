@@ -33,6 +33,7 @@ void set_error(const char* msg) { g_last_error = msg; }
 GcVariantInfo variant_info(int v) {
   GcVariantInfo x;
   if (v == 2) gc_variant_info_v2(&x);
+  else if (v == 3) gc_variant_info_v3(&x);
   else gc_variant_info_v1(&x);
   return x;
 }
@@ -74,7 +75,7 @@ Tuning current_tuning() {
         if (v[1] >= 0 && v[1] <= 32) g_tune.blocks_per_cu = v[1];
         if (v[2] >= 1 && v[2] <= 64) g_tune.cells_per_wave = v[2];
         if (v[3] >= 1 && v[3] <= 4096) g_tune.grain = v[3];
-        if (v[4] == 1 || v[4] == 2) g_tune.variant = v[4];
+        if (v[4] >= 1 && v[4] <= 3) g_tune.variant = v[4];
       }
     }
   }
@@ -183,6 +184,8 @@ hipError_t launch_chunk(DevCtx& c, int kernel, const Tuning& t, const GcBatchArg
   const int grain = (unsigned)t.grain > fair ? (int)fair : t.grain;
   if (t.variant == 2)
     return gc_launch_persistent_v2(&b, ws, S, q, t.block, t.cells_per_wave, grain, st);
+  if (t.variant == 3)
+    return gc_launch_persistent_v3(&b, ws, S, q, t.block, t.cells_per_wave, grain, st);
   return gc_launch_persistent_v1(&b, ws, S, q, t.block, t.cells_per_wave, grain, st);
 }
 
@@ -345,9 +348,9 @@ int record_solve(DevCtx& c, const GcBatchArgs& b, const GcRecord& rec_in, long* 
   unsigned fair = ((unsigned)b.nCell + waves - 1u) / waves;
   if (fair < 1u) fair = 1u;
   const int grain = (unsigned)t.grain > fair ? (int)fair : t.grain;
-  e = (t.variant == 2)
-          ? gc_launch_record_v2(&b, (double*)c.ws.p, S, q, t.block, t.cells_per_wave, grain, &rec, st)
-          : gc_launch_record_v1(&b, (double*)c.ws.p, S, q, t.block, t.cells_per_wave, grain, &rec, st);
+  e = (t.variant == 2)   ? gc_launch_record_v2(&b, (double*)c.ws.p, S, q, t.block, t.cells_per_wave, grain, &rec, st)
+      : (t.variant == 3) ? gc_launch_record_v3(&b, (double*)c.ws.p, S, q, t.block, t.cells_per_wave, grain, &rec, st)
+                         : gc_launch_record_v1(&b, (double*)c.ws.p, S, q, t.block, t.cells_per_wave, grain, &rec, st);
   if (e != hipSuccess) { set_error("launch", e); return -3; }
   if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
   unsigned counts[2] = {0, 0};
@@ -384,8 +387,9 @@ int adjoint_solve(DevCtx& c, const GcAdjArgs& a, hipStream_t st, double* tKernel
     return -2;
   }
   if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
-  e = (t.variant == 2) ? gc_launch_adjoint_v2(&aa, (double*)c.ws.p, 128, st)
-                       : gc_launch_adjoint_v1(&aa, (double*)c.ws.p, 128, st);
+  e = (t.variant == 2)   ? gc_launch_adjoint_v2(&aa, (double*)c.ws.p, 128, st)
+      : (t.variant == 3) ? gc_launch_adjoint_v3(&aa, (double*)c.ws.p, 128, st)
+                         : gc_launch_adjoint_v1(&aa, (double*)c.ws.p, 128, st);
   if (e != hipSuccess) { set_error("launch", e); return -3; }
   if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
   if ((e = hipStreamSynchronize(st)) != hipSuccess) { set_error("adjoint", e); return -3; }
@@ -638,7 +642,7 @@ int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave, int grai
   (void)current_tuning();  // apply GC_AMD_TUNE first so explicit values win
   if ((block > 0 && (block < 64 || block > 256 || block % 64 != 0)) ||
       (blocks_per_cu > 32) || (cells_per_wave > 64) || (grain > 4096) ||
-      (variant > 0 && variant != 1 && variant != 2))
+      (variant > 0 && variant > 3))
     return -1;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   if (block > 0) g_tune.block = block;

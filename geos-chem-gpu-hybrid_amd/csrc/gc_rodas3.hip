@@ -50,6 +50,10 @@
 #define GC_WPE 4
 #define GC_SEG 16
 #define GC_LU_TABLE gc_lu16
+#elif GC_VARIANT == 3
+#define GC_WPE 3
+#define GC_SEG 24
+#define GC_LU_TABLE gc_lu24
 #else
 #error "unknown GC_VARIANT"
 #endif

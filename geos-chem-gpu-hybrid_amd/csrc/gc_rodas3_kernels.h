@@ -93,3 +93,4 @@ struct GcAdjArgs {
 
 GC_DECLARE_VARIANT(1)
 GC_DECLARE_VARIANT(2)
+GC_DECLARE_VARIANT(3)

@@ -106,7 +106,8 @@ int gc_amd_integrate_batch_device(int kernel, int chunk, int nCell,
  *   cells_per_wave  lanes of each 64-wide wave that take cells (16..64)
  *   grain           queue positions a wave claims per atomic (1..4096)
  *   variant         LU row-segment variant: 1 = 32-entry segments in LDS
- *                   (2 waves per SIMD), 2 = 16-entry segments (4 waves per SIMD)
+ *                   (2 waves per SIMD), 2 = 16-entry segments (4 waves per
+ *                   SIMD), 3 = 24-entry segments (3 waves per SIMD)
  * Returns 0, or -1 if a value is out of range (nothing changed). */
 int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave,
                       int grain, int variant);

@@ -156,7 +156,7 @@ def test_tuning_variants_identical(cfg0, ref0, mech):
     lib = L.load()
     old = L.get_tuning(lib)
     try:
-        for t in ({"variant": 1}, {"variant": 2, "cells_per_wave": 32, "grain": 1},
+        for t in ({"variant": 1}, {"variant": 3}, {"variant": 2, "cells_per_wave": 32, "grain": 1},
                   {"block": 64, "blocks_per_cu": 4},
                   {"cells_per_wave": 16, "grain": 4096, "variant": 1, "blocks_per_cu": 0}):
             L.set_tuning(lib, **t)
