@@ -1,0 +1,94 @@
+"""Host-side logic that needs no GPU: the benchmark's one-GPU contract, the
+seeded input generators, the cost order, the autotuner's feasibility rule,
+the operation counts and the binding's argument checks."""
+import importlib.util
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import PKG, REPO
+from gcchem_amd import inputs, mechanism, solver
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, str(REPO / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr or "one MI355X" in r.stderr
+
+
+def test_cost_order_is_a_stable_descending_sort():
+    key = np.array([3, 7, 3, 1, 7, 0])
+    perm = inputs.cost_order(key)
+    assert perm.dtype == np.int32 and sorted(perm) == list(range(1, 7))
+    assert list(perm) == [2, 5, 1, 3, 4, 6]          # ties keep their order
+    assert list(inputs.cost_order(key, descending=False)) == [6, 4, 1, 3, 2, 5]
+
+
+def test_generators_are_seeded_and_shaped(mech):
+    a = inputs.config0(mech, n=16, seed=3)
+    b = inputs.config0(mech, n=16, seed=3)
+    c = inputs.config0(mech, n=16, seed=4)
+    for k in ("C", "RCONST", "ICNTRL", "RCNTRL"):
+        assert np.array_equal(a[k], b[k])
+    assert not np.array_equal(a["C"], c["C"])
+    assert a["C"].shape == (16, mech.nspec) and a["RCONST"].shape == (16, mech.nreact)
+    assert a["ICNTRL"].shape == (16, 20) and a["ICNTRL"].dtype == np.int32
+    assert list(a["ICNTRL"][0, [0, 1, 2, 14]]) == [1, 0, 4, -1]
+    # literal-rate reactions (A(126), A(735)) never read RCONST: it is 0 there
+    assert np.all(a["RCONST"][:, [125, 734]] == 0.0)
+    assert np.all(a["C"] > 0.0) and np.all(a["RCONST"] >= 0.0)
+
+
+def test_grid_and_adjoint_workloads(mech):
+    d = inputs.grid_config(mech, 4, 3, 5, seed=1)
+    assert d["C"].shape[0] == 4 * 3 * 5
+    d3 = inputs.config3(mech, n=100, seed=3)
+    assert d3["C"].shape == (100, mech.nspec) and d3["w"].shape == (mech.nvar,)
+    s = inputs.subset(d3, np.array([5, 2]))
+    assert np.array_equal(s["C"][1], d3["C"][2]) and list(s["Perm"]) == [1, 2]
+
+
+def test_stress_workload_caps_steps():
+    m = mechanism.load("stress500")
+    d = inputs.config4(m, 8, seed=4, max_steps=777)
+    assert np.all(d["ICNTRL"][:, 3] == 777)
+    assert d["C"].shape == (8, m.nspec)
+
+
+def test_autotune_feasibility():
+    spec = importlib.util.spec_from_file_location("autotune", PKG / "autotune.py")
+    at = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(at)
+    base = {"block": 256, "cells_per_wave": 64, "grain": 64}
+    assert at.feasible({**base, "variant": 1, "blocks_per_cu": 2})       # 8 waves, 135 KB LDS
+    assert not at.feasible({**base, "variant": 1, "blocks_per_cu": 3})   # 12 waves > target
+    assert at.feasible({**base, "variant": 2, "blocks_per_cu": 4})       # 16 waves, 139 KB
+    assert at.feasible({**base, "variant": 1, "blocks_per_cu": 0})       # library sizes it
+
+
+def test_operation_counts_reconcile_with_the_survey(mech):
+    """The generator's own counts of the reference expressions against the
+    survey's counts (Fun 11,139, Jac 10,607, LU 44,353, solve 11,013)."""
+    assert abs(mech.flops_fun() - 11139) / 11139 < 0.005
+    assert abs(mech.flops_jac() - 10607) / 10607 < 0.005
+    assert mech.flops_decomp() == 44353
+    assert mech.flops_solve() == 11013
+    istat = np.array([[10, 5, 6, 5, 1, 6, 24, 0]])
+    f = mech.algorithmic_flops(istat)
+    assert f == (mech.flops_fun() * 10 + mech.flops_jac() * 5
+                 + (mech.flops_decomp() + mech.flops_prepare()) * 6
+                 + mech.flops_solve() * 24 + mech.flops_step_vector() * 6)
+
+
+def test_solver_rejects_host_tensors_and_bad_shapes(mech):
+    d = inputs.config0(mech, n=4, seed=0)
+    b = solver.Batch.from_numpy(d, device="cpu")
+    with pytest.raises(TypeError):
+        solver._check(b, (mech.nvar, mech.nfix, mech.nreact, mech.nnz))
+    b.C = torch.zeros((4, mech.nspec + 1), dtype=torch.float64)
+    with pytest.raises((TypeError, ValueError)):
+        solver._check(b, (mech.nvar, mech.nfix, mech.nreact, mech.nnz))
