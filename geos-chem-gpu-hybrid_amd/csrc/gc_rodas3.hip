@@ -202,6 +202,16 @@ __device__ void run_products(const Blk& b, const int* table, const int* grp_) {
 __device__ __forceinline__ double sum_terms(const Blk& b, i4 first, ctab_t*& t, int n) {
   double acc = dbl(first.z, first.w) * b.ld(first.x);
   int k = 1;
+  for (; k + 8 <= n; k += 8, t += 32) {
+    i4 q[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) q[r] = ld4(t + 4 * r);
+    double x[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) x[r] = b.ld(q[r].x);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc = acc + dbl(q[r].z, q[r].w) * x[r];
+  }
   for (; k + 4 <= n; k += 4, t += 16) {
     const i4 q0 = ld4(t), q1 = ld4(t + 4), q2 = ld4(t + 8), q3 = ld4(t + 12);
     const double x0 = b.ld(q0.x), x1 = b.ld(q1.x), x2 = b.ld(q2.x), x3 = b.ld(q3.x);
@@ -300,8 +310,8 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
       b.lds_st(slot, g);
     } else if (code == 2 || code == 3) {
       // n updates {slot, U} (n a multiple of 4; padding entries update the
-      // spare slot), four per iteration; the first four are loaded together
-      // with the pivot operand
+      // spare slot, real ones have distinct slots): the first four are loaded
+      // together with the pivot operand, then eight per iteration, then four
       const int n = h.w;
       op += 4;
       i4 p01 = f, p23 = ld4(op + 4);
@@ -320,15 +330,45 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
       } else {
         a = -b.ld(h.y);
       }
-      for (int u = 0; u < n; u += 4, op += 8) {
-        if (u > 0) {
-          p01 = ld4(op);
-          p23 = ld4(op + 4);
-          u0 = b.ld(p01.y);
-          u1 = b.ld(p01.w);
-          u2 = b.ld(p23.y);
-          u3 = b.ld(p23.w);
+      int u = 0;
+      if (n > 0) {
+        const double w0 = b.lds_ld(p01.x), w1 = b.lds_ld(p01.z);
+        const double w2 = b.lds_ld(p23.x), w3 = b.lds_ld(p23.z);
+        b.lds_st(p01.x, w0 + a * u0);
+        b.lds_st(p01.z, w1 + a * u1);
+        b.lds_st(p23.x, w2 + a * u2);
+        b.lds_st(p23.z, w3 + a * u3);
+        u = 4;
+        op += 8;
+      }
+      for (; u + 8 <= n; u += 8, op += 16) {
+        i4 pp[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pp[r] = ld4(op + 4 * r);
+        double uu[8], ww[8];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          uu[2 * r] = b.ld(pp[r].y);
+          uu[2 * r + 1] = b.ld(pp[r].w);
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          ww[2 * r] = b.lds_ld(pp[r].x);
+          ww[2 * r + 1] = b.lds_ld(pp[r].z);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          b.lds_st(pp[r].x, ww[2 * r] + a * uu[2 * r]);
+          b.lds_st(pp[r].z, ww[2 * r + 1] + a * uu[2 * r + 1]);
+        }
+      }
+      for (; u < n; u += 4, op += 8) {
+        p01 = ld4(op);
+        p23 = ld4(op + 4);
+        u0 = b.ld(p01.y);
+        u1 = b.ld(p01.w);
+        u2 = b.ld(p23.y);
+        u3 = b.ld(p23.w);
         const double w0 = b.lds_ld(p01.x), w1 = b.lds_ld(p01.z);
         const double w2 = b.lds_ld(p23.x), w3 = b.lds_ld(p23.z);
         b.lds_st(p01.x, w0 + a * u0);
@@ -369,7 +409,23 @@ __device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows) {
       x = x - l3 * x3;
       t += 8;
     }
-    for (int u = 4; u < n; u += 4, t += 8) {
+    int u = 4;
+    for (; u + 8 <= n; u += 8, t += 16) {
+      i4 q[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) q[r] = ld4(t + 4 * r);
+      double l[8], y[8];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        l[2 * r] = b.ld(q[r].x);
+        y[2 * r] = b.ld(q[r].y);
+        l[2 * r + 1] = b.ld(q[r].z);
+        y[2 * r + 1] = b.ld(q[r].w);
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r) x = x - l[r] * y[r];
+    }
+    for (; u < n; u += 4, t += 8) {
       const i4 q01 = ld4(t), q23 = ld4(t + 4);
       const double l0 = b.ld(q01.x), x0 = b.ld(q01.y), l1 = b.ld(q01.z), x1 = b.ld(q01.w);
       const double l2 = b.ld(q23.x), x2 = b.ld(q23.y), l3 = b.ld(q23.z), x3 = b.ld(q23.w);
