@@ -165,6 +165,19 @@ __device__ __forceinline__ double prod_rec(const Blk& b, i4 a, i4 c) {
 template <int NF, bool LIT>
 __device__ __forceinline__ ctab_t* prod_group(const Blk& b, ctab_t* rec, int n) {
   int r = 0;
+  for (; r + 4 <= n; r += 4, rec += 32) {
+    i4 a[4], c[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = ld4(rec + 8 * q);
+      c[q] = ld4(rec + 8 * q + 4);
+    }
+    double v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = prod_rec<NF, LIT>(b, a[q], c[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b.st(a[q].x, v[q]);
+  }
   for (; r + 2 <= n; r += 2, rec += 16) {
     const i4 a0 = ld4(rec), c0 = ld4(rec + 4), a1 = ld4(rec + 8), c1 = ld4(rec + 12);
     const double v0 = prod_rec<NF, LIT>(b, a0, c0);
@@ -311,7 +324,7 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
     } else if (code == 2 || code == 3) {
       // n updates {slot, U} (n a multiple of 4; padding entries update the
       // spare slot, real ones have distinct slots): the first four are loaded
-      // together with the pivot operand, then eight per iteration, then four
+      // together with the pivot operand, then sixteen, eight, four per iteration
       const int n = h.w;
       op += 4;
       i4 p01 = f, p23 = ld4(op + 4);
@@ -340,6 +353,27 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
         b.lds_st(p23.z, w3 + a * u3);
         u = 4;
         op += 8;
+      }
+      for (; u + 16 <= n; u += 16, op += 32) {
+        i4 pp[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) pp[r] = ld4(op + 4 * r);
+        double uu[16], ww[16];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          uu[2 * r] = b.ld(pp[r].y);
+          uu[2 * r + 1] = b.ld(pp[r].w);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          ww[2 * r] = b.lds_ld(pp[r].x);
+          ww[2 * r + 1] = b.lds_ld(pp[r].z);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          b.lds_st(pp[r].x, ww[2 * r] + a * uu[2 * r]);
+          b.lds_st(pp[r].z, ww[2 * r + 1] + a * uu[2 * r + 1]);
+        }
       }
       for (; u + 8 <= n; u += 8, op += 16) {
         i4 pp[4];
