@@ -42,7 +42,9 @@ Tables (int32 words):
   Vdot(i) = P_VAR(i) - D_VAR(i)*V(i).  (gckpp_Function.F90:3252-5058)
 * ``lu<S>``: fused Jac_SP JVS sums + matrix preparation + KppDecomp with the
   active row segment (at most S entries, plus one spare slot) in LDS; 4-word
-  ops: ENTRY {1, slot, nterms, diag} + nterms terms; PIVL {2, slot_j, U(j,j),
+  ops: ENTRYG {5, T, 0, 0} + T terms {soff, flags, coef} holding every JVS
+  entry of the segment (flags: first/last term of an entry, diagonal, slot
+  index << 8; T a multiple of 4); PIVL {2, slot_j, U(j,j),
   nupd} and PIVS {3, L(k,j), 0, nupd} + nupd/2 4-word update pairs
   {t, U(j,c), t, U(j,c)} (nupd a multiple of 4); STORE {4, n, 0, 0} + n/2
   {slot, LU, slot, LU} (n a multiple of 2); END {0,...}.
@@ -209,6 +211,30 @@ class Tables:
         return out
 
     # -- Jacobian + LU ---------------------------------------------------------
+    def entry_group(self, entries, spare) -> list[int]:
+        """All JVS entries of one row segment as one ENTRYG record {5, T, 0, 0}
+        + T terms {soff, flags, coef_lo, coef_hi} (T a multiple of 4), the
+        kernel's loads going out eight or four terms at a time.  flags: 1 first
+        term of an entry, 2 last term, 4 diagonal entry, slot index << 8.  An
+        entry with no term (a structural zero of the LU pattern) is one term
+        1.0*ZERO, whose +0.0 is the empty sum; padding terms write the spare
+        slot."""
+        resolve = lambda x: self.data_off(x, "Y")
+        zero = self.off("ZERO", 0)
+        words = []
+        for sl, dg, terms in entries:
+            flags = (4 if dg else 0) | ((sl // 512) << 8)
+            if not terms:
+                words += [zero, flags | 3, *_coef_words(1.0)]
+                continue
+            for q, (sg, prod) in enumerate(terms):
+                w = self.sum_term(sg, prod, resolve)
+                w[1] = flags | (1 if q == 0 else 0) | (2 if q == len(terms) - 1 else 0)
+                words += w
+        while len(words) % 16:
+            words += [zero, 3 | ((spare // 512) << 8), *_coef_words(1.0)]
+        return [5, len(words) // 4, 0, 0] + words
+
     def lu(self, seg: int) -> list[int]:
         ir = self.ir
         nvar = ir["nvar"]
@@ -232,11 +258,8 @@ class Tables:
                 segcols = cols[s0:s0 + seg]
                 slot = {c: n * 512 for n, c in enumerate(segcols)}
                 lo = segcols[0]
-                for c in segcols:
-                    terms = jvs[pos[c]][2]
-                    ops += [1, slot[c], len(terms), 1 if c == k else 0]
-                    for s, p in terms:
-                        ops += self.sum_term(s, p, lambda x: self.data_off(x, "Y"))
+                ops += self.entry_group([(slot[c], c == k, jvs[pos[c]][2]) for c in segcols],
+                                        spare)
                 for kk in range(crow[k], diag[k]):
                     j = icol[kk]
                     if j > segcols[-1]:

@@ -295,6 +295,24 @@ __device__ void fun(const Blk& b, bool ynew, int out) {
   }
 }
 
+// One term {soff, flags, coef} of an ENTRYG record with its operand x: the
+// entry's sum restarts at a first term; at its last term the prepared entry
+// -sum (+ ghinv on the diagonal) goes to its LDS slot.
+__device__ __forceinline__ void entry_term(const Blk& b, i4 t, double x, double ghinv,
+                                           double& acc, int& sing) {
+  const int fl = t.y;
+  const double cx = dbl(t.z, t.w) * x;
+  acc = (fl & 1) ? cx : acc + cx;
+  if (fl & 2) {
+    double g = -acc;
+    if (fl & 4) {
+      g = g + ghinv;
+      sing |= (__builtin_fabs(g) < TINYD) ? 1 : 0;
+    }
+    b.lds_st((fl >> 8) * 512, g);
+  }
+}
+
 // Jac_SP fused with Ghimj = -Jac0, diag += ghinv and KppDecomp, row segments
 // in LDS (the lu table format is described in codegen/gen_tables.py).
 // Returns 1 when a prepared diagonal entry is below TINY.
@@ -321,6 +339,33 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
         sing |= (__builtin_fabs(g) < TINYD) ? 1 : 0;
       }
       b.lds_st(slot, g);
+    } else if (code == 5) {
+      // every entry of the segment: T terms (a multiple of 4), eight or four
+      // loads in flight at a time; an entry may span batches
+      const int T = h.y;
+      op += 4;
+      double acc = 0.0;
+      int q = 0;
+      for (; q + 8 <= T; q += 8, op += 32) {
+        i4 tt[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) tt[r] = ld4(op + 4 * r);
+        double x[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) x[r] = b.ld(tt[r].x);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) entry_term(b, tt[r], x[r], ghinv, acc, sing);
+      }
+      for (; q < T; q += 4, op += 16) {
+        i4 tt[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tt[r] = ld4(op + 4 * r);
+        double x[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[r] = b.ld(tt[r].x);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) entry_term(b, tt[r], x[r], ghinv, acc, sing);
+      }
     } else if (code == 2 || code == 3) {
       // n updates {slot, U} (n a multiple of 4; padding entries update the
       // spare slot, real ones have distinct slots): the first four are loaded

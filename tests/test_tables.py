@@ -129,6 +129,22 @@ class TableMachine:
                     g = g + ghinv
                     sing |= np.abs(g) < 2.2250738585072014e-308
                 self.lds[slot // 512] = g
+            elif code == 5:
+                T = w[p + 1]
+                assert T % 4 == 0
+                p += 4
+                acc = None
+                for q in range(T):
+                    soff, fl = w[p], w[p + 1]
+                    cx = _coef(w[p + 2], w[p + 3]) * self.ld(soff)
+                    acc = cx if fl & 1 else acc + cx
+                    if fl & 2:
+                        g = -acc
+                        if fl & 4:
+                            g = g + ghinv
+                            sing |= np.abs(g) < 2.2250738585072014e-308
+                        self.lds[fl >> 8] = g
+                    p += 4
             elif code in (2, 3):
                 if code == 2:
                     sj = w[p + 1]
