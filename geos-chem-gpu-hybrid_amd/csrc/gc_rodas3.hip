@@ -144,10 +144,6 @@ __device__ __forceinline__ double dbl(int lo, int hi) {
                                         ((unsigned long long)(unsigned)hi << 32));
 }
 
-__device__ __forceinline__ double coef_of(ctab_t* t) {
-  const unsigned long long lo = (unsigned)t[2], hi = (unsigned)t[3];
-  return __builtin_bit_cast(double, lo | (hi << 32));
-}
 
 // One product record {dst, lit, f0, f1, f2, f3, 0, 0} of shape (NF, LIT):
 // lit*f0*f1*... left to right.
@@ -324,22 +320,7 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
     const i4 h = ld4(op), f = ld4(op + 4);  // header and the first 4 words after it
     const int code = h.x;
     if (code == 0) break;
-    if (code == 1) {
-      const int slot = h.y, nt = h.z, isdiag = h.w;
-      op += 8;
-      double v = 0.0;
-      if (nt > 0) {
-        v = sum_terms(b, f, op, nt);
-      } else {
-        op -= 4;
-      }
-      double g = -v;
-      if (isdiag) {
-        g = g + ghinv;
-        sing |= (__builtin_fabs(g) < TINYD) ? 1 : 0;
-      }
-      b.lds_st(slot, g);
-    } else if (code == 5) {
+    if (code == 5) {
       // every entry of the segment: T terms (a multiple of 4), eight or four
       // loads in flight at a time; an entry may span batches
       const int T = h.y;
