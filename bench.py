@@ -58,7 +58,8 @@ def cpu_baseline(mech, d, perm, target_s=15.0):
     from oracle import Oracle
     from gcchem_amd import inputs
     o = Oracle(mech.ir, pow_mode="libm")
-    threads = min(os.cpu_count() or 1, 64)
+    # the host's share of cores for one GPU is 16 threads
+    threads = min(len(os.sched_getaffinity(0)) or 1, 16)
     n_all = d["C"].shape[0]
 
     def run(n):
@@ -79,7 +80,19 @@ def cpu_baseline(mech, d, perm, target_s=15.0):
     return {"value": round(n / secs, 1), "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": f"{n} cells evenly spaced in the launch order of the same {n_all}-cell "
                       f"workload, OpenMP schedule(dynamic,16) on {threads} threads, "
-                      f"{secs:.1f} s wall"}
+                      f"{secs:.1f} s wall",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "s_per_chem_step": round(n_all * secs / n, 1)}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def adjoint_ratio(mech, n=65536):
