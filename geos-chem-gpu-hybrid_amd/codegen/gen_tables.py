@@ -54,10 +54,23 @@ Tables (int32 words):
   vector): solve rows, header {n, x_i, U(i,i) or 0, 0} then n/2
   {LU, x_j, LU, x_j} with n a multiple of 4, absolute offsets.
   (gckpp_LinearAlgebra.F90:645-2311, 2327-3701)
-* discrete adjoint: ``adj_nu`` (nu = S^T mu per reaction, S the net
+* discrete adjoint: ``adj_nu`` (nu = S^T mu per reaction, 4-word terms
+  {TMP(i), NU(r) | flags, coef}, S the net
   stoichiometry of the Fun_SPLIT sums, cf. CalcStoichNum,
-  gckpp_Function.F90:7271-12790) and ``adj_mono_y``/``adj_mono_yn`` (the
-  rate monomials with their factor, direction and gradient offsets).
+  gckpp_Function.F90:7271-12790), then the derivatives of the rate monomials
+  A(r) = c*k_r*x1[*x2[*x3]] weighted by nu:
+  ``prod_adj_w`` (product records W(r) = c*nu(r)*k_r into A's elements, free
+  during an adjoint step), ``adj_gk1_y``/``adj_gk1_yn`` (first order, 8-word
+  records {GK(r), NU(r), c_lo, c_hi, x1, x2, x3, 0}: GK += (nu*c)*((x1*x2)*x3),
+  missing factors ONE), ``adj_gk2`` + ``adjgrp_gk2`` (second order along the
+  direction DIR, 12-word records {GK, NU, c_lo, c_hi, x1, x2, x3, d1, d2, d3,
+  0, 0} grouped by factor count) and the gather sums ``adj_lp1_y``/
+  ``adj_lp1_yn``/``adj_lp2``: per gradient target (LP(i) of every variable,
+  then GF(j) of the fixed species that have terms) 4-word terms {W | a<<16,
+  b | c<<16, d, dst | flags} (element indices), each adding W*((a*b)+(c*d)),
+  in reaction order, from init (ZERO for LP, GF itself).  ``adj_nu`` and the
+  gather sums are streams of terms across all their targets (flags mark the
+  first and last term of each target), so a batch of loads spans targets.
 """
 from __future__ import annotations
 
@@ -97,7 +110,8 @@ def layout(ir) -> dict:
     E["NU"] = E["DIR"] + nvar      # S^T mu per reaction
     E["GK"] = E["NU"] + nreact     # dL/dRCONST
     E["GF"] = E["GK"] + nreact     # dL/dFIX
-    E["ADJ_TOTAL"] = E["GF"] + nfix
+    E["NZERO"] = E["GF"] + nfix    # holds -0.0 (exact no-op term of the gather sums)
+    E["ADJ_TOTAL"] = E["NZERO"] + 1
     return E
 
 
@@ -136,7 +150,8 @@ class Tables:
 
     def data_off(self, f, vbase: str) -> int:
         kind, idx = f
-        return self.off({"V": vbase, "F": "FIX", "RCT": "RCT", "A": "A", "B": "B"}[kind], idx)
+        return self.off({"V": vbase, "F": "FIX", "RCT": "RCT", "A": "A", "B": "B",
+                         "NU": "NU"}[kind], idx)
 
     # -- products ------------------------------------------------------------
     def product_shape(self, prod, vbase):
@@ -391,6 +406,132 @@ class Tables:
                     *x, *d, *g, 0]
         return out
 
+    def adj_nu_stream(self):
+        """nu = S^T mu as one stream of 4-word terms {TMP(i), NU(r) | flags,
+        coef_lo, coef_hi} over all reactions in order (flags: 1 first term of
+        NU(r): the sum restarts at coef*mu; 2 last term: NU(r) is stored), a
+        multiple of 8 terms.  A reaction without terms stores 1.0*ZERO; padding
+        terms store into SINK."""
+        cols = [[] for _ in range(self.ir["nreact"])]
+        for i, row in enumerate(self.stoichiometry()):
+            for r, c in sorted(row.items()):
+                if c != 0.0:
+                    cols[r].append((self.off("TMP", i), c))
+        out = []
+        for r, col in enumerate(cols):
+            col = col or [(self.off("ZERO", 0), 1.0)]
+            for k, (x, c) in enumerate(col):
+                fl = (1 if k == 0 else 0) | (2 if k == len(col) - 1 else 0)
+                out += [x, self.off("NU", r) | fl, *_coef_words(c)]
+        while len(out) % 32:
+            out += [self.off("ZERO", 0), self.off("SINK", 0) | 3, *_coef_words(0.0)]
+        return out
+
+    def _monomial_parts(self, r, vbase):
+        """(c, has_k, factors) of A(r): the literal coefficient (1.0 if none)
+        and per data factor (value offset at vbase or FIX, direction component
+        DIR or ZERO, gradient target LP or GF)."""
+        lit, has_k, data = self.rate_monomial(r)
+        fac = []
+        for kind, idx in data:
+            if kind == "V":
+                fac.append((self.off(vbase, idx), self.off("DIR", idx), self.off("LP", idx)))
+            else:
+                fac.append((self.off("FIX", idx), self.off("ZERO", 0), self.off("GF", idx)))
+        return (1.0 if lit is None else lit), has_k, fac
+
+    def adj_w(self):
+        """W(r) = c*NU(r)*RCT(r) (the factor nu*c*k of every first- and
+        second-order term) as product records into A's elements."""
+        items = []
+        for r in range(self.ir["nreact"]):
+            lit, has_k, _ = self.rate_monomial(r)
+            prod = ([("lit", self._rate_lit_text(r))] if lit is not None else []) + [("NU", r)]
+            if has_k:
+                prod.append(("RCT", r))
+            items.append((prod, "Y", self.off("A", r)))
+        return self.product_group(items)
+
+    def adj_gk1(self, vbase):
+        """First-order dL/dk records: GK(r) += (nu*c)*((x1*x2)*x3)."""
+        one = self.off("ONE", 0)
+        out = []
+        for r in range(self.ir["nreact"]):
+            c, has_k, fac = self._monomial_parts(r, vbase)
+            if not has_k:
+                continue
+            x = [f[0] for f in fac] + [one] * (3 - len(fac))
+            out += [self.off("GK", r), self.off("NU", r), *_coef_words(c), *x, 0]
+        return out
+
+    def adj_gk2(self):
+        """Second-order dL/dk records along DIR, grouped by factor count
+        (1, 2, 3): GK(r) += (nu*c)*d1, (nu*c)*((x2*d1)+(x1*d2)),
+        (nu*c)*((((x2*x3)*d1)+((x1*x3)*d2))+((x1*x2)*d3))."""
+        z, one = self.off("ZERO", 0), self.off("ONE", 0)
+        groups = [[], [], []]
+        for r in range(self.ir["nreact"]):
+            c, has_k, fac = self._monomial_parts(r, "Y")
+            if not has_k:
+                continue
+            x = [f[0] for f in fac] + [one] * (3 - len(fac))
+            d = [f[1] for f in fac] + [z] * (3 - len(fac))
+            groups[len(fac) - 1].append([self.off("GK", r), self.off("NU", r), *_coef_words(c),
+                                         *x, *d, 0, 0])
+        return [w for g in groups for rec in g for w in rec], [len(g) for g in groups]
+
+    def adj_lp(self, second, vbase):
+        """Gather sums of the monomial derivatives into their gradient
+        targets: first order (d/dx_f of W*prod x) at vbase, or second order
+        (d/dx_f of W*sum_g prod_{h!=g} x_h d_g) at Y along DIR.  One stream of
+        4-word terms {W | a<<16, b | c<<16, d, dst | flags} (W..d element
+        indices), each W*((a*b)+(c*d)); per target its initial value (ZERO
+        for LP, GF itself) as the first term and then its terms in reaction
+        order (flags as adj_nu_stream); a multiple of 8 terms, padding terms
+        storing into SINK."""
+        one, nz = self.off("ONE", 0), self.off("NZERO", 0)
+        W = lambda r: self.off("A", r)
+        terms: dict[int, list] = {}
+        for r in range(self.ir["nreact"]):
+            _, _, fac = self._monomial_parts(r, "Y" if second else vbase)
+            nf = len(fac)
+            x = [f[0] for f in fac]
+            dd = [f[1] for f in fac]
+            for f in range(nf):
+                o = [h for h in range(nf) if h != f]   # the other factors, in order
+                if not second:
+                    if nf == 1:
+                        t = (one, one, nz, one)
+                    elif nf == 2:
+                        t = (x[o[0]], one, nz, one)
+                    else:
+                        t = (x[o[0]], x[o[1]], nz, one)
+                else:
+                    if nf == 1:
+                        continue
+                    if nf == 2:
+                        t = (dd[o[0]], one, nz, one)
+                    else:   # w*(x_b*d_a + x_a*d_b) for the others a < b
+                        a, b = o
+                        t = (x[b], dd[a], x[a], dd[b])
+                terms.setdefault(fac[f][2], []).append((W(r), *t))
+        if self.E["ADJ_TOTAL"] > 0xFFFF:
+            raise ValueError("element index beyond 16 bits in the gather terms")
+        e = lambda o: o // 512
+        out = []
+        targets = [self.off("LP", i) for i in range(self.ir["nvar"])]
+        targets += [g for g in (self.off("GF", j) for j in range(self.ir["nfix"])) if g in terms]
+        for g in targets:
+            init = self.off("ZERO", 0) if g < self.off("GF", 0) else g
+            # the initial value as a term: init*((1*1)+(-0*1)) == init
+            ts = [(init, one, one, nz, one)] + list(terms.get(g, []))
+            for k, (w, a, b, c, d) in enumerate(ts):
+                fl = (1 if k == 0 else 0) | (2 if k == len(ts) - 1 else 0)
+                out += [e(w) | e(a) << 16, e(b) | e(c) << 16, e(d), g | fl]
+        while len(out) % 32:
+            out += [e(nz) | e(one) << 16, e(one) | e(nz) << 16, e(one), self.off("SINK", 0) | 3]
+        return [v - (1 << 32) if v >= 1 << 31 else v for v in out]   # as int32
+
     # -- solves ----------------------------------------------------------------
     # One table per target vector, absolute offsets: padding pairs read ZERO
     # for both operands, so x - 0.0*0.0 = x exactly.
@@ -491,8 +632,20 @@ def generate(ir: dict) -> str:
         sizes[f"fwd{stage}"] = len(fw)
     tr1, n1, tr2, n2 = t.tr("TMP")
     body += [_arr("gc_tr1", tr1), _arr("gc_tr2", tr2)]
-    body += [_arr("gc_adj_nu", t.adj_nu()), _arr("gc_adj_mono_y", t.adj_mono("Y")),
-             _arr("gc_adj_mono_yn", t.adj_mono("YNEW"))]
+    body.append(_arr("gc_adj_nu", t.adj_nu_stream()))
+    words, counts = t.adj_w()
+    body += [_arr("gc_prod_adj_w", words), _arr("gc_prodgrp_adj_w", counts)]
+    body += [_arr("gc_adj_gk1_y", t.adj_gk1("Y")), _arr("gc_adj_gk1_yn", t.adj_gk1("YNEW"))]
+    words, counts = t.adj_gk2()
+    body += [_arr("gc_adj_gk2", words), _arr("gc_adjgrp_gk2", counts)]
+    nterm = {"NU": len(t.adj_nu_stream()) // 4}
+    for name, (second, vb) in {"lp1_y": (False, "Y"), "lp1_yn": (False, "YNEW"),
+                               "lp2": (True, "Y")}.items():
+        words = t.adj_lp(second, vb)
+        nterm[name.upper()] = len(words) // 4
+        body.append(_arr(f"gc_adj_{name}", words))
+    parts += [f"#define MECH_NGK {len(t.adj_gk1('Y')) // 8}"]
+    parts += [f"#define MECH_NT_{k} {v}" for k, v in nterm.items()]
     for s in SEGMENTS:
         ops = t.lu(s)
         body.append(_arr(f"gc_lu{s}", ops))

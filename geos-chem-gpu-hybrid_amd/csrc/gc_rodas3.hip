@@ -131,6 +131,27 @@ __device__ __forceinline__ Blk make_blk(double* ws, unsigned wave, int lane, dou
   return b;
 }
 
+// Element passes: val(k) loads (and combines) what element k needs, put(k, v)
+// stores it.  VB consecutive elements are evaluated, their loads issued
+// together, before any of them is stored, so a pass keeps VB elements' loads
+// in flight instead of one round trip per element (the stores of a pass would
+// otherwise order every later load behind them).  Valid for passes in which
+// element k reads nothing that the pass stores at another index.
+constexpr int VB = 8;
+
+template <int N, class V, class P>
+__device__ __forceinline__ void vpass(V val, P put) {
+  int k = 0;
+  for (; k + VB <= N; k += VB) {
+    decltype(val(0)) v[VB];
+#pragma unroll
+    for (int u = 0; u < VB; ++u) v[u] = val(k + u);
+#pragma unroll
+    for (int u = 0; u < VB; ++u) put(k + u, v[u]);
+  }
+  for (; k < N; ++k) put(k, val(k));
+}
+
 // Table words arrive in 16-byte groups (one s_load_dwordx4 each): every
 // record starts on a 4-word boundary of a 64-byte aligned table, and every
 // table has 16 words of zero padding after its last record.
@@ -234,12 +255,6 @@ __device__ __forceinline__ double sum_terms(const Blk& b, i4 first, ctab_t*& t, 
     acc = acc + dbl(q.z, q.w) * b.ld(q.x);
   }
   return acc;
-}
-
-__device__ __forceinline__ double sum_terms(const Blk& b, ctab_t*& t, int n) {
-  const i4 first = ld4(t);
-  t += 4;
-  return sum_terms(b, first, t, n);
 }
 
 // One D_VAR term {f1, f2, f3, ci}: ((lits[ci]*f1)*f2)*f3, the signed literal
@@ -546,9 +561,9 @@ __device__ int decode_opts(const int* icu, const double* rcu, double Tstart, dou
 __device__ void gather(const GcBatchArgs& a, long cell, const Blk& b) {
   const double* c = a.C + cell * NSPEC;
   const double* r = a.RCONST + cell * NREACT;
-  for (int i = 0; i < NVAR; ++i) b.st(O_Y + i * 512, c[i]);
+  vpass<NVAR>([&](int i) { return c[i]; }, [&](int i, double v) { b.st(O_Y + i * 512, v); });
   for (int j = 0; j < NFIX; ++j) b.st(O_FIX + j * 512, c[NVAR + j]);
-  for (int k = 0; k < NREACT; ++k) b.st(O_RCT + k * 512, r[k]);
+  vpass<NREACT>([&](int k) { return r[k]; }, [&](int k, double v) { b.st(O_RCT + k * 512, v); });
   b.st(MECH_E_ZERO * 512, 0.0);  // padding operand of the solve tables
   b.st(MECH_E_ONE * 512, 1.0);   // missing factor of a D_VAR term
 }
@@ -556,7 +571,7 @@ __device__ void gather(const GcBatchArgs& a, long cell, const Blk& b) {
 __device__ void scatter(const GcBatchArgs& a, long cell, const Blk& b, const int* ist,
                         const double* rst, int ierr) {
   double* c = a.C + cell * NSPEC;
-  for (int i = 0; i < NVAR; ++i) c[i] = b.ld(O_Y + i * 512);
+  vpass<NVAR>([&](int i) { return b.ld(O_Y + i * 512); }, [&](int i, double v) { c[i] = v; });
   for (int k = 0; k < 8; ++k) a.ISTAT[cell * 8 + k] = ist[k];
   for (int k = 0; k < 3; ++k) a.RSTAT[cell * 3 + k] = rst[k];
   a.IERR[cell] = ierr;
@@ -571,18 +586,21 @@ __device__ void step_functions(const Blk& b, const Opts& o, double T, int* ist) 
     const double Delta = sqrt(ROUNDOFF) * dmax(1.0e-6, fabs(T));
     fun(b, false, O_DFDT);
     ist[0] += 1;
-    for (int k = 0; k < NVAR; ++k)
-      b.st(O_DFDT + k * 512, (b.ld(O_DFDT + k * 512) - b.ld(O_FCN0 + k * 512)) / Delta);
+    vpass<NVAR>(
+        [&](int k) { return (b.ld(O_DFDT + k * 512) - b.ld(O_FCN0 + k * 512)) / Delta; },
+        [&](int k, double v) { b.st(O_DFDT + k * 512, v); });
   }
 }
 
 // Ynew = Y + sum_j A_sj K_j, the point of stage s >= 2 (ros_NewF).
 __device__ void stage_point(const Blk& b, int s) {
-  for (int k = 0; k < NVAR; ++k) {
-    double y = b.ld(O_Y + k * 512);
-    for (int j = 0; j < s; ++j) y = y + ros_A[s * (s - 1) / 2 + j] * b.ld(O_K + j * O_VEC + k * 512);
-    b.st(O_YNEW + k * 512, y);
-  }
+  vpass<NVAR>(
+      [&](int k) {
+        double y = b.ld(O_Y + k * 512);
+        for (int j = 0; j < s; ++j) y = y + ros_A[s * (s - 1) / 2 + j] * b.ld(O_K + j * O_VEC + k * 512);
+        return y;
+      },
+      [&](int k, double y) { b.st(O_YNEW + k * 512, y); });
 }
 
 // The four stages after a successful factorisation
@@ -603,15 +621,17 @@ __device__ void stages(const Blk& b, bool autonomous, double Dir, double H, int*
     }
     double HC[3];
     for (int j = 0; j < s; ++j) HC[j] = ros_C[s * (s - 1) / 2 + j] / (Dir * H);
-    for (int k = 0; k < NVAR; ++k) {
-      double v = b.ld(src + k * 512);
-      for (int j = 0; j < s; ++j) v = v + HC[j] * b.ld(O_K + j * O_VEC + k * 512);
-      b.st(Ks + k * 512, v);
-    }
+    vpass<NVAR>(
+        [&](int k) {
+          double v = b.ld(src + k * 512);
+          for (int j = 0; j < s; ++j) v = v + HC[j] * b.ld(O_K + j * O_VEC + k * 512);
+          return v;
+        },
+        [&](int k, double v) { b.st(Ks + k * 512, v); });
     if (!autonomous && ros_Gamma[s] != 0.0) {
       const double HG = (Dir * H) * ros_Gamma[s];
-      for (int k = 0; k < NVAR; ++k)
-        b.st(Ks + k * 512, b.ld(Ks + k * 512) + HG * b.ld(O_DFDT + k * 512));
+      vpass<NVAR>([&](int k) { return b.ld(Ks + k * 512) + HG * b.ld(O_DFDT + k * 512); },
+                  [&](int k, double v) { b.st(Ks + k * 512, v); });
     }
     solve(b, s);
     ist[6] += 1;
@@ -624,26 +644,37 @@ __device__ double stages_and_error(const Blk& b, const Opts& o, double Dir, doub
                                    const double* atol, const double* rtol, int* ist) {
   stages(b, o.autonomous, Dir, H, ist);
   double Err = 0.0;
-  for (int k = 0; k < NVAR; ++k) {
-    const double y = b.ld(O_Y + k * 512);
-    double yn = y, ye = 0.0;
-    for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * b.ld(O_K + j * O_VEC + k * 512);
-    for (int j = 0; j < 4; ++j) ye = ye + ros_E[j] * b.ld(O_K + j * O_VEC + k * 512);
-    b.st(O_YNEW + k * 512, yn);
-    const double Ymax = dmax(fabs(y), fabs(yn));
-    const double Scal = o.vector_tol ? atol[k] + rtol[k] * Ymax : atol[0] + rtol[0] * Ymax;
-    const double q = ye / Scal;
-    Err = Err + q * q;
-  }
+  struct YE {
+    double y, yn, ye;
+  };
+  vpass<NVAR>(
+      [&](int k) {
+        const double y = b.ld(O_Y + k * 512);
+        double k4[4];
+        for (int j = 0; j < 4; ++j) k4[j] = b.ld(O_K + j * O_VEC + k * 512);
+        double yn = y, ye = 0.0;
+        for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * k4[j];
+        for (int j = 0; j < 4; ++j) ye = ye + ros_E[j] * k4[j];
+        return YE{y, yn, ye};
+      },
+      [&](int k, YE v) {  // Err accumulates in element order
+        b.st(O_YNEW + k * 512, v.yn);
+        const double Ymax = dmax(fabs(v.y), fabs(v.yn));
+        const double Scal = o.vector_tol ? atol[k] + rtol[k] * Ymax : atol[0] + rtol[0] * Ymax;
+        const double q = v.ye / Scal;
+        Err = Err + q * q;
+      });
   Err = sqrt(Err / (double)NVAR);
   return dmax(Err, 1.0e-10);
 }
 
 __device__ void accept_y(const Blk& b, const Opts& o) {
   if (o.clamp) {
-    for (int k = 0; k < NVAR; ++k) b.st(O_Y + k * 512, dmax(b.ld(O_YNEW + k * 512), 0.0));
+    vpass<NVAR>([&](int k) { return dmax(b.ld(O_YNEW + k * 512), 0.0); },
+                [&](int k, double v) { b.st(O_Y + k * 512, v); });
   } else {
-    for (int k = 0; k < NVAR; ++k) b.st(O_Y + k * 512, b.ld(O_YNEW + k * 512));
+    vpass<NVAR>([&](int k) { return b.ld(O_YNEW + k * 512); },
+                [&](int k, double v) { b.st(O_Y + k * 512, v); });
   }
 }
 
@@ -758,7 +789,7 @@ __device__ __forceinline__ void record_step(const Blk& b, const GcRecord& rec, b
   int yslot = -1;
   if (ck && ys < rec.ycap) {
     double* y = rec.Y + (size_t)ys * NVAR;
-    for (int k = 0; k < NVAR; ++k) y[k] = b.ld(O_Y + k * 512);
+    vpass<NVAR>([&](int k) { return b.ld(O_Y + k * 512); }, [&](int k, double v) { y[k] = v; });
     yslot = (int)ys;
   }
   rec.H[slot] = H;
@@ -961,73 +992,171 @@ __device__ void solve_tr(const Blk& b) {
   solve_rows<false>(b, tab(gc_tr2), MECH_NTR2);
 }
 
-// nu(r) = sum_i S(i,r) mu(i) for mu in TMP, S the net stoichiometry.
+// A term of a sum stream: the sum restarts at a first term (flags & 1) and is
+// stored at dst when its last term (flags & 2) is added; word = dst | flags.
+__device__ __forceinline__ void stream_term(const Blk& b, int word, double v, double& acc) {
+  acc = (word & 1) ? v : acc + v;
+  if (word & 2) b.st(word & ~511, acc);
+}
+
+// nu(r) = sum_i S(i,r) mu(i) for mu in TMP, S the net stoichiometry: one
+// stream of terms {TMP(i), NU(r) | flags, coef} over all reactions, eight
+// loads in flight per batch.
 __device__ void adj_nu(const Blk& b) {
   ctab_t* t = tab(gc_adj_nu);
-  for (int r = 0; r < NREACT; ++r) {
-    const int n = t[0], dst = t[1];
-    t += 4;
-    b.st(dst, n > 0 ? sum_terms(b, t, n) : 0.0);
+  double acc = 0.0;
+  for (int q = 0; q < MECH_NT_NU; q += 8, t += 32) {
+    i4 tt[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) tt[r] = ld4(t + 4 * r);
+    double x[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) x[r] = b.ld(tt[r].x);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) stream_term(b, tt[r].y, dbl(tt[r].z, tt[r].w) * x[r], acc);
   }
 }
 
-__device__ __forceinline__ void acc(const Blk& b, int g, double v) { b.st(g, b.ld(g) + v); }
+// Derivatives of the rate monomials A_r = c k_r x1 [x2 [x3]] weighted by
+// nu_r, W_r = c nu_r k_r (in A's elements, free during the adjoint step):
+//  first order  : dL/dk_r += (nu c) prod x;          g_f += W prod_{h != f} x_h
+//  second order : the gradient of nu_r (grad A_r . d) for the direction d in
+//                 DIR: dL/dk_r += (nu c) sum_g prod_{h != g} x_h d_g,
+//                 g_f += W d/dx_f (sum_g prod_{h != g} x_h d_g)
+// with g_f the gradient target of factor f (LP for variables, GF for fixed
+// species).  The k_r records have distinct targets, four per iteration; the
+// g_f targets are gathered, each from its terms in reaction order.
 
-// Rate monomials A_r = c k x1 [x2 [x3]] with weights nu_r:
-//  first order  (SECOND false): g_f += nu c k prod_{h != f} x_h, GK_r += nu c prod x
-//  second order (SECOND true) : the gradient of nu_r (grad A_r . d) for the
-//               direction d in DIR: g_f += nu c k d/dx_f (sum_g prod_{h != g} x_h d_g),
-//               GK_r += nu c sum_g prod_{h != g} x_h d_g
-// Targets g are LP (variables) or GF (fixed species); see codegen adj_mono.
-template <bool SECOND>
-__device__ void adj_mono(const Blk& b, const int* table) {
+// first order: GK += (nu*c)*((x1*x2)*x3), records {GK, NU, c, c, x1, x2, x3, 0}
+__device__ void adj_gk1(const Blk& b, const int* table) {
   ctab_t* q = tab(table);
-  for (int r = 0; r < NREACT; ++r, q += 16) {
-    const int nf = q[0], hk = q[1], li = q[2];
-    const double nu = b.ld(q[3]);
-    const double c = li >= 0 ? ((clit_t*)gc_lits)[li] : 1.0;
-    const double k = hk ? b.ld(q[4]) : 1.0;
-    const double x1 = b.ld(q[6]);
-    const double w = nu * c * k;
-    if (!SECOND) {
-      if (nf == 1) {
-        acc(b, q[12], w);
-        if (hk) acc(b, q[5], nu * c * x1);
-      } else if (nf == 2) {
-        const double x2 = b.ld(q[7]);
-        acc(b, q[12], w * x2);
-        acc(b, q[13], w * x1);
-        if (hk) acc(b, q[5], nu * c * (x1 * x2));
-      } else {
-        const double x2 = b.ld(q[7]), x3 = b.ld(q[8]);
-        acc(b, q[12], w * (x2 * x3));
-        acc(b, q[13], w * (x1 * x3));
-        acc(b, q[14], w * (x1 * x2));
-        if (hk) acc(b, q[5], nu * c * (x1 * x2 * x3));
-      }
-    } else {
-      const double d1 = b.ld(q[9]);
-      if (nf == 1) {
-        if (hk) acc(b, q[5], nu * c * d1);
-      } else if (nf == 2) {
-        const double x2 = b.ld(q[7]), d2 = b.ld(q[10]);
-        acc(b, q[12], w * d2);
-        acc(b, q[13], w * d1);
-        if (hk) acc(b, q[5], nu * c * (x2 * d1 + x1 * d2));
-      } else {
-        const double x2 = b.ld(q[7]), x3 = b.ld(q[8]);
-        const double d2 = b.ld(q[10]), d3 = b.ld(q[11]);
-        acc(b, q[12], w * (x3 * d2 + x2 * d3));
-        acc(b, q[13], w * (x3 * d1 + x1 * d3));
-        acc(b, q[14], w * (x2 * d1 + x1 * d2));
-        if (hk) acc(b, q[5], nu * c * (x2 * x3 * d1 + x1 * x3 * d2 + x1 * x2 * d3));
-      }
+  int r = 0;
+  for (; r + 4 <= MECH_NGK; r += 4, q += 32) {
+    i4 h[4], f[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      h[u] = ld4(q + 8 * u);
+      f[u] = ld4(q + 8 * u + 4);
     }
+    double g[4], nu[4], x1[4], x2[4], x3[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      g[u] = b.ld(h[u].x);
+      nu[u] = b.ld(h[u].y);
+      x1[u] = b.ld(f[u].x);
+      x2[u] = b.ld(f[u].y);
+      x3[u] = b.ld(f[u].z);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      b.st(h[u].x, g[u] + (nu[u] * dbl(h[u].z, h[u].w)) * ((x1[u] * x2[u]) * x3[u]));
+  }
+  for (; r < MECH_NGK; ++r, q += 8) {
+    const i4 h = ld4(q), f = ld4(q + 4);
+    const double v = (b.ld(h.y) * dbl(h.z, h.w)) * ((b.ld(f.x) * b.ld(f.y)) * b.ld(f.z));
+    b.st(h.x, b.ld(h.x) + v);
   }
 }
 
-__device__ __forceinline__ void vzero(const Blk& b, int o) {
-  for (int k = 0; k < NVAR; ++k) b.st(o + k * 512, 0.0);
+// second order, records {GK, NU, c, c, x1, x2, x3, d1, d2, d3, 0, 0} of NF factors
+template <int NF>
+__device__ __forceinline__ double gk2_term(const Blk& b, i4 f, i4 e) {
+  const double d1 = b.ld(f.w);
+  if (NF == 1) return d1;
+  const double x1 = b.ld(f.x), x2 = b.ld(f.y), d2 = b.ld(e.x);
+  if (NF == 2) return (x2 * d1) + (x1 * d2);
+  const double x3 = b.ld(f.z), d3 = b.ld(e.y);
+  return (((x2 * x3) * d1) + ((x1 * x3) * d2)) + ((x1 * x2) * d3);
+}
+
+template <int NF>
+__device__ __forceinline__ ctab_t* gk2_group(const Blk& b, ctab_t* q, int n) {
+  int r = 0;
+  for (; r + 4 <= n; r += 4, q += 48) {
+    i4 h[4], f[4], e[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      h[u] = ld4(q + 12 * u);
+      f[u] = ld4(q + 12 * u + 4);
+      e[u] = ld4(q + 12 * u + 8);
+    }
+    double g[4], nu[4], p[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      g[u] = b.ld(h[u].x);
+      nu[u] = b.ld(h[u].y);
+      p[u] = gk2_term<NF>(b, f[u], e[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) b.st(h[u].x, g[u] + (nu[u] * dbl(h[u].z, h[u].w)) * p[u]);
+  }
+  for (; r < n; ++r, q += 12) {
+    const i4 h = ld4(q), f = ld4(q + 4), e = ld4(q + 8);
+    const double v = (b.ld(h.y) * dbl(h.z, h.w)) * gk2_term<NF>(b, f, e);
+    b.st(h.x, b.ld(h.x) + v);
+  }
+  return q;
+}
+
+__device__ void adj_gk2(const Blk& b) {
+  ctab_t* q = tab(gc_adj_gk2);
+  q = gk2_group<1>(b, q, gc_adjgrp_gk2[0]);
+  q = gk2_group<2>(b, q, gc_adjgrp_gk2[1]);
+  gk2_group<3>(b, q, gc_adjgrp_gk2[2]);
+}
+
+// Gather sums: one stream of terms {W | a<<16, b | c<<16, d, dst | flags}
+// (W..d element indices) over all targets, each term W*((a*b)+(c*d)); the
+// first term of a target is its initial value.  Eight terms per batch.
+__device__ __forceinline__ int eoff(int w, int sh) { return ((w >> sh) & 0xFFFF) * 512; }
+
+__device__ void adj_lp(const Blk& b, const int* table, int nterm) {
+  ctab_t* t = tab(table);
+  double acc = 0.0;
+  for (int q = 0; q < nterm; q += 8, t += 32) {
+    i4 tt[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) tt[r] = ld4(t + 4 * r);
+    double w[8], xa[8], xb[8], xc[8], xd[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      w[r] = b.ld(eoff(tt[r].x, 0));
+      xa[r] = b.ld(eoff(tt[r].x, 16));
+      xb[r] = b.ld(eoff(tt[r].y, 0));
+      xc[r] = b.ld(eoff(tt[r].y, 16));
+      xd[r] = b.ld(eoff(tt[r].z, 0));
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      stream_term(b, tt[r].w, w[r] * ((xa[r] * xb[r]) + (xc[r] * xd[r])), acc);
+  }
+}
+
+// GY += LP and lambda_Kj += C_sj/(Dir*H) mu_s (+ A_sj LP for s >= 2), j < S.
+template <int S>
+__device__ __forceinline__ void adj_update(const Blk& b, double dh) {
+  struct U {
+    double lp, mu, gy, lk[S > 0 ? S : 1];
+  };
+  vpass<NVAR>(
+      [&](int k) {
+        U u;
+        u.lp = b.ld(O_LP + k * 512);
+        u.mu = b.ld(O_TMP + k * 512);
+        u.gy = b.ld(O_GY + k * 512);
+#pragma unroll
+        for (int j = 0; j < S; ++j) u.lk[j] = b.ld(O_LK + j * O_VEC + k * 512);
+        return u;
+      },
+      [&](int k, const U& u) {
+        b.st(O_GY + k * 512, u.gy + u.lp);
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+          double v = u.lk[j] + (ros_C[S * (S - 1) / 2 + j] / dh) * u.mu;
+          if (S >= 2) v = v + ros_A[S * (S - 1) / 2 + j] * u.lp;
+          b.st(O_LK + j * O_VEC + k * 512, v);
+        }
+      });
 }
 
 // One accepted step in reverse.  On entry Y holds the step's start state, the
@@ -1037,44 +1166,49 @@ __device__ __forceinline__ void vzero(const Blk& b, int o) {
 __device__ void adj_step(const Blk& b, double Dir, double H, bool clamp) {
   const double dh = Dir * H;
   if (clamp) {  // Y' = max(Ynew, 0): no sensitivity where Ynew <= 0
-    for (int k = 0; k < NVAR; ++k) {
-      double yn = b.ld(O_Y + k * 512);
-      for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * b.ld(O_K + j * O_VEC + k * 512);
-      if (!(yn > 0.0)) b.st(O_LAM + k * 512, 0.0);
-    }
+    vpass<NVAR>(
+        [&](int k) {
+          double yn = b.ld(O_Y + k * 512);
+          for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * b.ld(O_K + j * O_VEC + k * 512);
+          return yn;
+        },
+        [&](int k, double yn) {
+          if (!(yn > 0.0)) b.st(O_LAM + k * 512, 0.0);
+        });
   }
-  for (int k = 0; k < NVAR; ++k) {
-    const double l = b.ld(O_LAM + k * 512);
-    for (int j = 0; j < 4; ++j) b.st(O_LK + j * O_VEC + k * 512, ros_M[j] * l);
-    b.st(O_GY + k * 512, l);
-  }
+  vpass<NVAR>([&](int k) { return b.ld(O_LAM + k * 512); },
+              [&](int k, double l) {
+                for (int j = 0; j < 4; ++j) b.st(O_LK + j * O_VEC + k * 512, ros_M[j] * l);
+                b.st(O_GY + k * 512, l);
+              });
   for (int s = 3; s >= 0; --s) {
     // mu_s = M^-T lambda_Ks, nu = S^T mu_s
-    for (int k = 0; k < NVAR; ++k) b.st(O_TMP + k * 512, b.ld(O_LK + s * O_VEC + k * 512));
+    vpass<NVAR>([&](int k) { return b.ld(O_LK + s * O_VEC + k * 512); },
+                [&](int k, double v) { b.st(O_TMP + k * 512, v); });
     solve_tr(b);
     adj_nu(b);
     // first order at the stage point: J(P_s)^T mu_s, d/dk, d/dF
     if (s >= 2) stage_point(b, s);
-    vzero(b, O_LP);
-    adj_mono<false>(b, s >= 2 ? gc_adj_mono_yn : gc_adj_mono_y);
-    for (int k = 0; k < NVAR; ++k) {
-      const double lp = b.ld(O_LP + k * 512), mu = b.ld(O_TMP + k * 512);
-      b.st(O_GY + k * 512, b.ld(O_GY + k * 512) + lp);
-      for (int j = 0; j < s; ++j) {
-        const int o = O_LK + j * O_VEC + k * 512;
-        double v = b.ld(o) + (ros_C[s * (s - 1) / 2 + j] / dh) * mu;
-        if (s >= 2) v = v + ros_A[s * (s - 1) / 2 + j] * lp;
-        b.st(o, v);
-      }
+    run_products(b, gc_prod_adj_w, gc_prodgrp_adj_w);
+    adj_gk1(b, s >= 2 ? gc_adj_gk1_yn : gc_adj_gk1_y);
+    if (s >= 2) adj_lp(b, gc_adj_lp1_yn, MECH_NT_LP1_YN);
+    else adj_lp(b, gc_adj_lp1_y, MECH_NT_LP1_Y);
+    switch (s) {
+      case 3: adj_update<3>(b, dh); break;
+      case 2: adj_update<2>(b, dh); break;
+      case 1: adj_update<1>(b, dh); break;
+      default: adj_update<0>(b, dh); break;
     }
     // second order: d/dY of mu_s^T J(Y) K_s
-    for (int k = 0; k < NVAR; ++k) b.st(O_DIR + k * 512, b.ld(O_K + s * O_VEC + k * 512));
-    vzero(b, O_LP);
-    adj_mono<true>(b, gc_adj_mono_y);
-    for (int k = 0; k < NVAR; ++k)
-      b.st(O_GY + k * 512, b.ld(O_GY + k * 512) + b.ld(O_LP + k * 512));
+    vpass<NVAR>([&](int k) { return b.ld(O_K + s * O_VEC + k * 512); },
+                [&](int k, double v) { b.st(O_DIR + k * 512, v); });
+    adj_gk2(b);
+    adj_lp(b, gc_adj_lp2, MECH_NT_LP2);
+    vpass<NVAR>([&](int k) { return b.ld(O_GY + k * 512) + b.ld(O_LP + k * 512); },
+                [&](int k, double v) { b.st(O_GY + k * 512, v); });
   }
-  for (int k = 0; k < NVAR; ++k) b.st(O_LAM + k * 512, b.ld(O_GY + k * 512));
+  vpass<NVAR>([&](int k) { return b.ld(O_GY + k * 512); },
+              [&](int k, double v) { b.st(O_LAM + k * 512, v); });
 }
 
 }  // namespace
@@ -1110,10 +1244,11 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
     return;
   }
   for (int j = 0; j < NFIX; ++j) b.st(O_FIX + j * 512, c[NVAR + j]);
-  for (int r = 0; r < NREACT; ++r) b.st(O_RCT + r * 512, rc[r]);
+  vpass<NREACT>([&](int r) { return rc[r]; }, [&](int r, double v) { b.st(O_RCT + r * 512, v); });
   b.st(MECH_E_ZERO * 512, 0.0);
   b.st(MECH_E_ONE * 512, 1.0);
-  for (int i = 0; i < NVAR; ++i) b.st(O_LAM + i * 512, lam[i]);
+  b.st(MECH_E_NZERO * 512, -0.0);
+  vpass<NVAR>([&](int i) { return lam[i]; }, [&](int i, double v) { b.st(O_LAM + i * 512, v); });
   for (int j = 0; j < NFIX; ++j) b.st(O_GF + j * 512, lam[NVAR + j]);
   for (int r = 0; r < NREACT; ++r) b.st(O_GK + r * 512, 0.0);
   const double Dir = (a.Tend >= a.Tstart) ? 1.0 : -1.0;
@@ -1152,9 +1287,10 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
     // keeping the start state of each of the segment's steps; SEGH holds the
     // step sizes last step first
     const double* y0 = a.recY + (size_t)ys * NVAR;
-    for (int i = 0; i < NVAR; ++i) b.st(O_Y + i * 512, y0[i]);
+    vpass<NVAR>([&](int i) { return y0[i]; }, [&](int i, double v) { b.st(O_Y + i * 512, v); });
     for (int q = 0; q < len; ++q) {
-      for (int i = 0; i < NVAR; ++i) b.st(O_SEG + (q * NVAR + i) * 512, b.ld(O_Y + i * 512));
+      vpass<NVAR>([&](int i) { return b.ld(O_Y + i * 512); },
+                  [&](int i, double v) { b.st(O_SEG + (q * NVAR + i) * 512, v); });
       if (q == len - 1) break;
       const double H = b.ld(O_SEGH + (len - 1 - q) * 512);
       fun(b, false, O_FCN0);
@@ -1163,17 +1299,20 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
         break;
       }
       stages(b, true, Dir, H, ist);
-      for (int k = 0; k < NVAR; ++k) {  // Ynew and the accepted update, as the stepper
-        double yn = b.ld(O_Y + k * 512);
-        for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * b.ld(O_K + j * O_VEC + k * 512);
-        b.st(O_Y + k * 512, clamp ? dmax(yn, 0.0) : yn);
-      }
+      vpass<NVAR>(  // Ynew and the accepted update, as the stepper
+          [&](int k) {
+            double yn = b.ld(O_Y + k * 512);
+            for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * b.ld(O_K + j * O_VEC + k * 512);
+            return clamp ? dmax(yn, 0.0) : yn;
+          },
+          [&](int k, double v) { b.st(O_Y + k * 512, v); });
     }
     if (ierr != 1) break;
     // the adjoint through the segment's steps, last first
     for (int q = len - 1; q >= 0; --q) {
       const double H = b.ld(O_SEGH + (len - 1 - q) * 512);
-      for (int i = 0; i < NVAR; ++i) b.st(O_Y + i * 512, b.ld(O_SEG + (q * NVAR + i) * 512));
+      vpass<NVAR>([&](int i) { return b.ld(O_SEG + (q * NVAR + i) * 512); },
+                  [&](int i, double v) { b.st(O_Y + i * 512, v); });
       fun(b, false, O_FCN0);
       if (jac_lu(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
         ierr = -8;
@@ -1183,9 +1322,9 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
       adj_step(b, Dir, H, clamp);
     }
   }
-  for (int i = 0; i < NVAR; ++i) gc[i] = b.ld(O_LAM + i * 512);
+  vpass<NVAR>([&](int i) { return b.ld(O_LAM + i * 512); }, [&](int i, double v) { gc[i] = v; });
   for (int j = 0; j < NFIX; ++j) gc[NVAR + j] = b.ld(O_GF + j * 512);
-  for (int r = 0; r < NREACT; ++r) gk[r] = b.ld(O_GK + r * 512);
+  vpass<NREACT>([&](int r) { return b.ld(O_GK + r * 512); }, [&](int r, double v) { gk[r] = v; });
   a.IERR[cell] = ierr;
 }
 

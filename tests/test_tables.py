@@ -47,11 +47,19 @@ class TableMachine:
         self.bwd = [self.t.bwd("K", s) for s in range(4)]
         self.tr = self.t.tr("TMP")
         self.nu_tab = self.t.adj_nu()
+        self.nu_stream = self.t.adj_nu_stream()
         self.mono = {"Y": self.t.adj_mono("Y"), "YNEW": self.t.adj_mono("YNEW")}
+        self.prod["adj_w"] = self.t.adj_w()
+        self.gk1 = {"Y": self.t.adj_gk1("Y"), "YNEW": self.t.adj_gk1("YNEW")}
+        self.gk2 = self.t.adj_gk2()
+        self.lp = {(False, "Y"): self.t.adj_lp(False, "Y"),
+                   (False, "YNEW"): self.t.adj_lp(False, "YNEW"),
+                   (True, "Y"): self.t.adj_lp(True, "Y")}
         self.lits = np.array(self.t.lits or [0.0])
         self.ws = np.full((self.E["ADJ_TOTAL"], lanes), np.nan)
         self.ws[self.E["ZERO"]] = 0.0
         self.ws[self.E["ONE"]] = 1.0
+        self.ws[self.E["NZERO"]] = -0.0
         self.lds = np.full((seg + 1, lanes), np.nan)
 
     # workspace access by byte offset e*512 (a lane's column)
@@ -244,7 +252,66 @@ class TableMachine:
     def acc(self, g, v):
         self.st(g, self.ld(g) + v)
 
+    def adj_gk_first(self, point):
+        w = self.gk1[point]
+        for p in range(0, len(w), 8):
+            g, nu, lo, hi, x1, x2, x3 = w[p:p + 7]
+            v = (self.ld(nu) * _coef(lo, hi)) * ((self.ld(x1) * self.ld(x2)) * self.ld(x3))
+            self.st(g, self.ld(g) + v)
+
+    def adj_gk_second(self):
+        w, counts = self.gk2
+        p = 0
+        for nf, n in zip((1, 2, 3), counts):
+            for _ in range(n):
+                g, nu, lo, hi, x1, x2, x3, d1, d2, d3 = w[p:p + 10]
+                L = self.ld
+                if nf == 1:
+                    t = L(d1)
+                elif nf == 2:
+                    t = (L(x2) * L(d1)) + (L(x1) * L(d2))
+                else:
+                    t = (((L(x2) * L(x3)) * L(d1)) + ((L(x1) * L(x3)) * L(d2))) + ((L(x1) * L(x2)) * L(d3))
+                self.st(g, self.ld(g) + (L(nu) * _coef(lo, hi)) * t)
+                p += 12
+        assert p == len(w)
+
+    def stream_term(self, word, v, acc):
+        acc = v if word & 1 else acc + v
+        if word & 2:
+            self.st(word & ~511, acc)
+        return acc
+
+    def adj_nu_stream(self):
+        w = self.nu_stream
+        assert len(w) % 32 == 0
+        acc = None
+        for p in range(0, len(w), 4):
+            acc = self.stream_term(w[p + 1], _coef(w[p + 2], w[p + 3]) * self.ld(w[p]), acc)
+
+    def adj_lp(self, second, point):
+        w = self.lp[(second, point)]
+        assert len(w) % 32 == 0
+        e = lambda v, sh: ((v >> sh) & 0xFFFF) * 512
+        acc = None
+        for p in range(0, len(w), 4):
+            x, y, z, word = w[p:p + 4]
+            v = self.ld(e(x, 0)) * ((self.ld(e(x, 16)) * self.ld(e(y, 0)))
+                                    + (self.ld(e(y, 16)) * self.ld(e(z, 0))))
+            acc = self.stream_term(word, v, acc)
+
+    def adj_mono_gather(self, second, point):
+        """The kernel's form: W, then the k records, then the gather sums."""
+        if not second:
+            self.products("adj_w")
+            self.adj_gk_first(point)
+        else:
+            self.adj_gk_second()
+        self.adj_lp(second, point)
+
     def adj_mono(self, second, point):
+        """Scatter form of the same derivatives (per reaction, accumulating
+        into LP/GF/GK), kept to check the gather form bit for bit."""
         q = self.mono[point]
         for r in range(self.ir["nreact"]):
             rec = q[16 * r:16 * r + 16]
@@ -286,6 +353,8 @@ class TableMachine:
                     self.acc(rec[14], w * (x2 * d1 + x1 * d2))
                     if hk: self.acc(rec[5], nu * c * (x2 * x3 * d1 + x1 * x3 * d2 + x1 * x2 * d3))
 
+    scatter = False  # True: the per-reaction scatter form of the monomial derivatives
+
     def adj_step(self, dh):
         """Reverse sweep of one step; dh (scalar or per lane) is Dir*H."""
         n = self.ir["nvar"]
@@ -298,12 +367,18 @@ class TableMachine:
             for k in range(n):
                 self.st(self.o("TMP", k), self.ld(self.o("LK", s * n + k)).copy())
             self.solve_tr()
-            self.adj_nu()
+            if self.scatter:
+                self.adj_nu()
+            else:
+                self.adj_nu_stream()
             if s >= 2:
                 self.stage_point(s)
-            for k in range(n):
-                self.st(self.o("LP", k), 0.0 * self.ld(self.o("ZERO")))
-            self.adj_mono(False, "YNEW" if s >= 2 else "Y")
+            if self.scatter:
+                for k in range(n):
+                    self.st(self.o("LP", k), 0.0 * self.ld(self.o("ZERO")))
+                self.adj_mono(False, "YNEW" if s >= 2 else "Y")
+            else:
+                self.adj_mono_gather(False, "YNEW" if s >= 2 else "Y")
             for k in range(n):
                 lp, mu = self.ld(self.o("LP", k)), self.ld(self.o("TMP", k))
                 self.acc(self.o("GY", k), lp)
@@ -315,8 +390,12 @@ class TableMachine:
                     self.st(o, v)
             for k in range(n):
                 self.st(self.o("DIR", k), self.ld(self.o("K", s * n + k)).copy())
-                self.st(self.o("LP", k), 0.0 * self.ld(self.o("ZERO")))
-            self.adj_mono(True, "Y")
+            if self.scatter:
+                for k in range(n):
+                    self.st(self.o("LP", k), 0.0 * self.ld(self.o("ZERO")))
+                self.adj_mono(True, "Y")
+            else:
+                self.adj_mono_gather(True, "Y")
             for k in range(n):
                 self.acc(self.o("GY", k), self.ld(self.o("LP", k)))
         for k in range(n):
@@ -474,6 +553,38 @@ def test_adjoint_step_tables_match_reference(name, checkforms):
             assert np.max(np.abs(got - ref)) <= 1e-9 * scale
             rel = np.abs(got - ref) / (np.abs(ref) + 1e-300 * scale)
             assert np.mean(rel <= 1e-10) >= 0.95 or ref.size < 10
+
+
+@pytest.mark.parametrize("name", ["fullchem", "stress500"])
+def test_adjoint_gather_equals_scatter(name):
+    """The kernel's gather form of the monomial derivatives (W records, k
+    records, per-target sums in reaction order) gives the per-reaction scatter
+    form's gradients bit for bit, on one adjoint step at random states."""
+    from gcchem_amd import mechanism
+    mech = mechanism.load(name)
+    ir = mech.ir
+    n, nf, nr = ir["nvar"], ir["nfix"], ir["nreact"]
+    lanes = 4
+    V, F, R = _cells(mech, lanes, 3)
+    out = []
+    for scatter in (True, False):
+        m = TableMachine(ir, lanes, 16)
+        m.scatter = scatter
+        _load(m, V, F, R)
+        E = m.E
+        rs = np.random.default_rng(9)
+        m.ws[E["LAM"]:E["LAM"] + n] = rs.standard_normal((n, lanes))
+        m.ws[E["GF"]:E["GF"] + nf] = rs.standard_normal((nf, lanes))
+        m.ws[E["GK"]:E["GK"] + nr] = 0.0
+        for s in range(4):   # stage vectors and factors of a nearby matrix
+            m.ws[E["K"] + s * n:E["K"] + (s + 1) * n] = rs.standard_normal((n, lanes)) * V.T * 1e-3
+        m.fun(False, m.o("FCN0"))
+        assert not m.jac_lu(np.full(lanes, 1e3)).any()
+        m.adj_step(np.full(lanes, 0.02))
+        out.append(m.ws[E["LAM"]:E["ADJ_TOTAL"]].copy())
+    a, b = out
+    assert np.isfinite(a).all()
+    assert np.array_equal(a.view(np.int64), b.view(np.int64))
 
 
 def test_stoichiometry_matches_calcstoichnum(mech, checkforms):
