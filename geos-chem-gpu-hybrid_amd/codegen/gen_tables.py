@@ -59,10 +59,10 @@ Tables (int32 words):
   stoichiometry of the Fun_SPLIT sums, cf. CalcStoichNum,
   gckpp_Function.F90:7271-12790), then the derivatives of the rate monomials
   A(r) = c*k_r*x1[*x2[*x3]] weighted by nu:
-  ``prod_adj_w`` (product records W(r) = c*nu(r)*k_r into A's elements, free
-  during an adjoint step), ``adj_gk1_y``/``adj_gk1_yn`` (first order, 8-word
-  records {GK(r), NU(r), c_lo, c_hi, x1, x2, x3, 0}: GK += (nu*c)*((x1*x2)*x3),
-  missing factors ONE), ``adj_gk2`` + ``adjgrp_gk2`` (second order along the
+  ``adj_wk_y``/``adj_wk_yn`` (per reaction W(r) = (c*nu(r))*k_r into A's
+  elements, free during an adjoint step, and the first-order dL/dk
+  GK += (c*nu)*((x1*x2)*x3); 4-word records of element indices, see
+  Tables.adj_wk), ``adj_gk2`` + ``adjgrp_gk2`` (second order along the
   direction DIR, 12-word records {GK, NU, c_lo, c_hi, x1, x2, x3, d1, d2, d3,
   0, 0} grouped by factor count) and the gather sums ``adj_lp1_y``/
   ``adj_lp1_yn``/``adj_lp2``: per gradient target (LP(i) of every variable,
@@ -150,8 +150,7 @@ class Tables:
 
     def data_off(self, f, vbase: str) -> int:
         kind, idx = f
-        return self.off({"V": vbase, "F": "FIX", "RCT": "RCT", "A": "A", "B": "B",
-                         "NU": "NU"}[kind], idx)
+        return self.off({"V": vbase, "F": "FIX", "RCT": "RCT", "A": "A", "B": "B"}[kind], idx)
 
     # -- products ------------------------------------------------------------
     def product_shape(self, prod, vbase):
@@ -440,35 +439,40 @@ class Tables:
                 fac.append((self.off("FIX", idx), self.off("ZERO", 0), self.off("GF", idx)))
         return (1.0 if lit is None else lit), has_k, fac
 
-    def adj_w(self):
-        """W(r) = c*NU(r)*RCT(r) (the factor nu*c*k of every first- and
-        second-order term) as product records into A's elements."""
-        items = []
-        for r in range(self.ir["nreact"]):
-            lit, has_k, _ = self.rate_monomial(r)
-            prod = ([("lit", self._rate_lit_text(r))] if lit is not None else []) + [("NU", r)]
-            if has_k:
-                prod.append(("RCT", r))
-            items.append((prod, "Y", self.off("A", r)))
-        return self.product_group(items)
-
-    def adj_gk1(self, vbase):
-        """First-order dL/dk records: GK(r) += (nu*c)*((x1*x2)*x3)."""
-        one = self.off("ONE", 0)
+    def adj_wk(self, vbase):
+        """W(r) = (c*NU(r))*k_r into A's elements together with the first-order
+        dL/dk record GK(r) += (c*NU(r))*((x1*x2)*x3); c*NU(r) is left in NU(r)
+        for the second-order records.  4-word records of
+        element indices {A(r) | GK(r)<<16, NU(r) | K<<16, x1 | x2<<16,
+        x3 | lit<<16} (K = RCT(r), or ONE without a rate constant, whose GK
+        is SINK; missing factors ONE; lit indexes gc_lits, 1.0 without a
+        literal), a multiple of 8 records (padding ones write SINK)."""
+        if self.E["ADJ_TOTAL"] > 0xFFFF:
+            raise ValueError("element index beyond 16 bits in the W records")
+        e = lambda o: o // 512
+        one, sink = self.off("ONE", 0), self.off("SINK", 0)
         out = []
         for r in range(self.ir["nreact"]):
             c, has_k, fac = self._monomial_parts(r, vbase)
-            if not has_k:
-                continue
             x = [f[0] for f in fac] + [one] * (3 - len(fac))
-            out += [self.off("GK", r), self.off("NU", r), *_coef_words(c), *x, 0]
-        return out
+            gk = self.off("GK", r) if has_k else sink
+            kx = self.off("RCT", r) if has_k else one
+            out += [e(self.off("A", r)) | e(gk) << 16, e(self.off("NU", r)) | e(kx) << 16,
+                    e(x[0]) | e(x[1]) << 16, e(x[2]) | self.lit_idx(c) << 16]
+        while len(out) % 32:
+            out += [e(sink) | e(sink) << 16, e(sink) | e(one) << 16,
+                    e(one) | e(one) << 16, e(one) | self.lit_idx(1.0) << 16]
+        return [v - (1 << 32) if v >= 1 << 31 else v for v in out]
 
     def adj_gk2(self):
         """Second-order dL/dk records along DIR, grouped by factor count
-        (1, 2, 3): GK(r) += (nu*c)*d1, (nu*c)*((x2*d1)+(x1*d2)),
-        (nu*c)*((((x2*x3)*d1)+((x1*x3)*d2))+((x1*x2)*d3))."""
-        z, one = self.off("ZERO", 0), self.off("ONE", 0)
+        (1, 2, 3): GK(r) += CN*d1, CN*((x2*d1)+(x1*d2)),
+        CN*((((x2*x3)*d1)+((x1*x3)*d2))+((x1*x2)*d3)), CN = c*nu(r) (left in
+        NU(r) by the W records); 4-word records of element indices
+        {GK | NU<<16, x1 | x2<<16, x3 | d1<<16, d2 | d3<<16}, each group a
+        multiple of 8 records (padding ones write SINK)."""
+        e = lambda o: o // 512
+        z, one, sink = self.off("ZERO", 0), self.off("ONE", 0), self.off("SINK", 0)
         groups = [[], [], []]
         for r in range(self.ir["nreact"]):
             c, has_k, fac = self._monomial_parts(r, "Y")
@@ -476,9 +480,15 @@ class Tables:
                 continue
             x = [f[0] for f in fac] + [one] * (3 - len(fac))
             d = [f[1] for f in fac] + [z] * (3 - len(fac))
-            groups[len(fac) - 1].append([self.off("GK", r), self.off("NU", r), *_coef_words(c),
-                                         *x, *d, 0, 0])
-        return [w for g in groups for rec in g for w in rec], [len(g) for g in groups]
+            groups[len(fac) - 1].append([e(self.off("GK", r)) | e(self.off("NU", r)) << 16,
+                                         e(x[0]) | e(x[1]) << 16, e(x[2]) | e(d[0]) << 16,
+                                         e(d[1]) | e(d[2]) << 16])
+        for g in groups:
+            while len(g) % 8:
+                g.append([e(sink) | e(z) << 16, e(one) | e(one) << 16, e(one) | e(z) << 16,
+                          e(z) | e(z) << 16])
+        words = [w for g in groups for rec in g for w in rec]
+        return [v - (1 << 32) if v >= 1 << 31 else v for v in words], [len(g) for g in groups]
 
     def adj_lp(self, second, vbase):
         """Gather sums of the monomial derivatives into their gradient
@@ -633,9 +643,8 @@ def generate(ir: dict) -> str:
     tr1, n1, tr2, n2 = t.tr("TMP")
     body += [_arr("gc_tr1", tr1), _arr("gc_tr2", tr2)]
     body.append(_arr("gc_adj_nu", t.adj_nu_stream()))
-    words, counts = t.adj_w()
-    body += [_arr("gc_prod_adj_w", words), _arr("gc_prodgrp_adj_w", counts)]
-    body += [_arr("gc_adj_gk1_y", t.adj_gk1("Y")), _arr("gc_adj_gk1_yn", t.adj_gk1("YNEW"))]
+    wk_y, wk_yn = t.adj_wk("Y"), t.adj_wk("YNEW")
+    body += [_arr("gc_adj_wk_y", wk_y), _arr("gc_adj_wk_yn", wk_yn)]
     words, counts = t.adj_gk2()
     body += [_arr("gc_adj_gk2", words), _arr("gc_adjgrp_gk2", counts)]
     nterm = {"NU": len(t.adj_nu_stream()) // 4}
@@ -644,7 +653,7 @@ def generate(ir: dict) -> str:
         words = t.adj_lp(second, vb)
         nterm[name.upper()] = len(words) // 4
         body.append(_arr(f"gc_adj_{name}", words))
-    parts += [f"#define MECH_NGK {len(t.adj_gk1('Y')) // 8}"]
+    parts += [f"#define MECH_NWK {len(wk_y) // 4}"]
     parts += [f"#define MECH_NT_{k} {v}" for k, v in nterm.items()]
     for s in SEGMENTS:
         ops = t.lu(s)

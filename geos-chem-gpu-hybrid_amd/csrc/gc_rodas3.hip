@@ -1017,6 +1017,9 @@ __device__ void adj_nu(const Blk& b) {
   }
 }
 
+// Byte offset of the element index in bits sh..sh+15 of a table word.
+__device__ __forceinline__ int eoff(int w, int sh) { return ((w >> sh) & 0xFFFF) * 512; }
+
 // Derivatives of the rate monomials A_r = c k_r x1 [x2 [x3]] weighted by
 // nu_r, W_r = c nu_r k_r (in A's elements, free during the adjoint step):
 //  first order  : dL/dk_r += (nu c) prod x;          g_f += W prod_{h != f} x_h
@@ -1024,76 +1027,36 @@ __device__ void adj_nu(const Blk& b) {
 //                 DIR: dL/dk_r += (nu c) sum_g prod_{h != g} x_h d_g,
 //                 g_f += W d/dx_f (sum_g prod_{h != g} x_h d_g)
 // with g_f the gradient target of factor f (LP for variables, GF for fixed
-// species).  The k_r records have distinct targets, four per iteration; the
-// g_f targets are gathered, each from its terms in reaction order.
+// species).  The k_r records have distinct targets; the g_f targets are
+// gathered, each from its terms in reaction order.
 
-// first order: GK += (nu*c)*((x1*x2)*x3), records {GK, NU, c, c, x1, x2, x3, 0}
-__device__ void adj_gk1(const Blk& b, const int* table) {
-  ctab_t* q = tab(table);
-  int r = 0;
-  for (; r + 4 <= MECH_NGK; r += 4, q += 32) {
-    i4 h[4], f[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      h[u] = ld4(q + 8 * u);
-      f[u] = ld4(q + 8 * u + 4);
-    }
-    double g[4], nu[4], x1[4], x2[4], x3[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      g[u] = b.ld(h[u].x);
-      nu[u] = b.ld(h[u].y);
-      x1[u] = b.ld(f[u].x);
-      x2[u] = b.ld(f[u].y);
-      x3[u] = b.ld(f[u].z);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      b.st(h[u].x, g[u] + (nu[u] * dbl(h[u].z, h[u].w)) * ((x1[u] * x2[u]) * x3[u]));
-  }
-  for (; r < MECH_NGK; ++r, q += 8) {
-    const i4 h = ld4(q), f = ld4(q + 4);
-    const double v = (b.ld(h.y) * dbl(h.z, h.w)) * ((b.ld(f.x) * b.ld(f.y)) * b.ld(f.z));
-    b.st(h.x, b.ld(h.x) + v);
-  }
-}
-
-// second order, records {GK, NU, c, c, x1, x2, x3, d1, d2, d3, 0, 0} of NF factors
+// second order, records of element indices {GK | CN<<16, x1 | x2<<16,
+// x3 | d1<<16, d2 | d3<<16} of NF factors, CN = c*nu: GK += CN*(sum_g prod_{h!=g} x_h d_g)
 template <int NF>
-__device__ __forceinline__ double gk2_term(const Blk& b, i4 f, i4 e) {
-  const double d1 = b.ld(f.w);
+__device__ __forceinline__ double gk2_term(const Blk& b, i4 f) {
+  const double d1 = b.ld(eoff(f.z, 16));
   if (NF == 1) return d1;
-  const double x1 = b.ld(f.x), x2 = b.ld(f.y), d2 = b.ld(e.x);
+  const double x1 = b.ld(eoff(f.y, 0)), x2 = b.ld(eoff(f.y, 16)), d2 = b.ld(eoff(f.w, 0));
   if (NF == 2) return (x2 * d1) + (x1 * d2);
-  const double x3 = b.ld(f.z), d3 = b.ld(e.y);
+  const double x3 = b.ld(eoff(f.z, 0)), d3 = b.ld(eoff(f.w, 16));
   return (((x2 * x3) * d1) + ((x1 * x3) * d2)) + ((x1 * x2) * d3);
 }
 
 template <int NF>
 __device__ __forceinline__ ctab_t* gk2_group(const Blk& b, ctab_t* q, int n) {
-  int r = 0;
-  for (; r + 4 <= n; r += 4, q += 48) {
-    i4 h[4], f[4], e[4];
+  for (int r = 0; r < n; r += 8, q += 32) {
+    i4 f[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      h[u] = ld4(q + 12 * u);
-      f[u] = ld4(q + 12 * u + 4);
-      e[u] = ld4(q + 12 * u + 8);
-    }
-    double g[4], nu[4], p[4];
+    for (int u = 0; u < 8; ++u) f[u] = ld4(q + 4 * u);
+    double g[8], cn[8], p[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      g[u] = b.ld(h[u].x);
-      nu[u] = b.ld(h[u].y);
-      p[u] = gk2_term<NF>(b, f[u], e[u]);
+    for (int u = 0; u < 8; ++u) {
+      g[u] = b.ld(eoff(f[u].x, 0));
+      cn[u] = b.ld(eoff(f[u].x, 16));
+      p[u] = gk2_term<NF>(b, f[u]);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) b.st(h[u].x, g[u] + (nu[u] * dbl(h[u].z, h[u].w)) * p[u]);
-  }
-  for (; r < n; ++r, q += 12) {
-    const i4 h = ld4(q), f = ld4(q + 4), e = ld4(q + 8);
-    const double v = (b.ld(h.y) * dbl(h.z, h.w)) * gk2_term<NF>(b, f, e);
-    b.st(h.x, b.ld(h.x) + v);
+    for (int u = 0; u < 8; ++u) b.st(eoff(f[u].x, 0), g[u] + cn[u] * p[u]);
   }
   return q;
 }
@@ -1108,8 +1071,6 @@ __device__ void adj_gk2(const Blk& b) {
 // Gather sums: one stream of terms {W | a<<16, b | c<<16, d, dst | flags}
 // (W..d element indices) over all targets, each term W*((a*b)+(c*d)); the
 // first term of a target is its initial value.  Eight terms per batch.
-__device__ __forceinline__ int eoff(int w, int sh) { return ((w >> sh) & 0xFFFF) * 512; }
-
 __device__ void adj_lp(const Blk& b, const int* table, int nterm) {
   ctab_t* t = tab(table);
   double acc = 0.0;
@@ -1129,6 +1090,35 @@ __device__ void adj_lp(const Blk& b, const int* table, int nterm) {
 #pragma unroll
     for (int r = 0; r < 8; ++r)
       stream_term(b, tt[r].w, w[r] * ((xa[r] * xb[r]) + (xc[r] * xd[r])), acc);
+  }
+}
+
+// W(r) = (c*nu)*k into A(r) with the first-order dL/dk GK(r) += (c*nu)*((x1*x2)*x3),
+// leaving c*nu in NU(r): records of element indices {A | GK<<16, NU | K<<16,
+// x1 | x2<<16, x3 | lit<<16}, eight per batch (distinct targets but SINK).
+__device__ void adj_wk(const Blk& b, const int* table) {
+  ctab_t* t = tab(table);
+  for (int q = 0; q < MECH_NWK; q += 8, t += 32) {
+    i4 tt[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) tt[r] = ld4(t + 4 * r);
+    double g[8], nu[8], k[8], x1[8], x2[8], x3[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      g[r] = b.ld(eoff(tt[r].x, 16));
+      nu[r] = b.ld(eoff(tt[r].y, 0));
+      k[r] = b.ld(eoff(tt[r].y, 16));
+      x1[r] = b.ld(eoff(tt[r].z, 0));
+      x2[r] = b.ld(eoff(tt[r].z, 16));
+      x3[r] = b.ld(eoff(tt[r].w, 0));
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const double cn = ((clit_t*)gc_lits)[(tt[r].w >> 16) & 0xFFFF] * nu[r];
+      b.st(eoff(tt[r].x, 0), cn * k[r]);
+      b.st(eoff(tt[r].x, 16), g[r] + cn * ((x1[r] * x2[r]) * x3[r]));
+      b.st(eoff(tt[r].y, 0), cn);  // c*nu for the second-order records
+    }
   }
 }
 
@@ -1189,8 +1179,7 @@ __device__ void adj_step(const Blk& b, double Dir, double H, bool clamp) {
     adj_nu(b);
     // first order at the stage point: J(P_s)^T mu_s, d/dk, d/dF
     if (s >= 2) stage_point(b, s);
-    run_products(b, gc_prod_adj_w, gc_prodgrp_adj_w);
-    adj_gk1(b, s >= 2 ? gc_adj_gk1_yn : gc_adj_gk1_y);
+    adj_wk(b, s >= 2 ? gc_adj_wk_yn : gc_adj_wk_y);
     if (s >= 2) adj_lp(b, gc_adj_lp1_yn, MECH_NT_LP1_YN);
     else adj_lp(b, gc_adj_lp1_y, MECH_NT_LP1_Y);
     switch (s) {

@@ -49,8 +49,21 @@ def lib_path(mech: str = "fullchem") -> Path:
     return LIB_DIR / f"libgcchemgpu_{mech}.so"
 
 
+def _torch_runtime_first() -> None:
+    """Import torch, when it is installed, before the library is loaded: torch
+    ships its own HIP runtime, and the library's libamdhip64 dependency then
+    resolves to that same instance, so device pointers of torch tensors and
+    torch's current device are valid inside the library.  (Loaded first, the
+    library would bring a second runtime into the process.)"""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 @lru_cache(maxsize=None)
 def load(mech: str = "fullchem") -> ctypes.CDLL:
+    _torch_runtime_first()
     path = lib_path(mech)
     if not path.exists():
         raise RuntimeError(

@@ -53,6 +53,41 @@ def ref0(cfg0, oracle_cr):
     return _run_oracle(oracle_cr, cfg0)
 
 
+_LOAD_BEFORE_TORCH = """
+import importlib.util, sys
+pkg = sys.argv[1]
+spec = importlib.util.spec_from_file_location("gcchem_amd", pkg + "/__init__.py",
+                                              submodule_search_locations=[pkg])
+mod = importlib.util.module_from_spec(spec)
+sys.modules["gcchem_amd"] = mod
+spec.loader.exec_module(mod)
+assert "torch" not in sys.modules
+from gcchem_amd import lib as L
+L.load("fullchem")                      # the library first, torch not imported yet
+from gcchem_amd import inputs, mechanism, solver
+import torch
+b = solver.Batch.from_numpy(inputs.config0(mechanism.load("fullchem"), n=64, seed=1))
+r = solver.solve(b, kernel="persistent")
+torch.cuda.synchronize()
+assert bool((r.IERR == 1).all())
+print("solved", int(r.ISTAT[:, 2].sum()))
+"""
+
+
+def test_library_loaded_before_torch():
+    """Loading the library before torch still leaves one HIP runtime in the
+    process (the binding imports torch first), so a solve on torch tensors
+    works; a second runtime would not know torch's device."""
+    import subprocess
+    import sys
+    from conftest import PKG
+    print("case: library loaded before torch, 64 cells", flush=True)
+    r = subprocess.run([sys.executable, "-c", _LOAD_BEFORE_TORCH, str(PKG)],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "solved" in r.stdout
+
+
 def test_selftest_and_info():
     lib = L.load()
     nd = ctypes.c_int()

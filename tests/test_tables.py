@@ -49,8 +49,7 @@ class TableMachine:
         self.nu_tab = self.t.adj_nu()
         self.nu_stream = self.t.adj_nu_stream()
         self.mono = {"Y": self.t.adj_mono("Y"), "YNEW": self.t.adj_mono("YNEW")}
-        self.prod["adj_w"] = self.t.adj_w()
-        self.gk1 = {"Y": self.t.adj_gk1("Y"), "YNEW": self.t.adj_gk1("YNEW")}
+        self.wk = {"Y": self.t.adj_wk("Y"), "YNEW": self.t.adj_wk("YNEW")}
         self.gk2 = self.t.adj_gk2()
         self.lp = {(False, "Y"): self.t.adj_lp(False, "Y"),
                    (False, "YNEW"): self.t.adj_lp(False, "YNEW"),
@@ -252,28 +251,36 @@ class TableMachine:
     def acc(self, g, v):
         self.st(g, self.ld(g) + v)
 
-    def adj_gk_first(self, point):
-        w = self.gk1[point]
-        for p in range(0, len(w), 8):
-            g, nu, lo, hi, x1, x2, x3 = w[p:p + 7]
-            v = (self.ld(nu) * _coef(lo, hi)) * ((self.ld(x1) * self.ld(x2)) * self.ld(x3))
-            self.st(g, self.ld(g) + v)
+    def adj_wk(self, point):
+        w = self.wk[point]
+        assert len(w) % 32 == 0
+        e = lambda v, sh: ((v >> sh) & 0xFFFF) * 512
+        for p in range(0, len(w), 4):
+            a, b, c, d = w[p:p + 4]
+            cn = self.lits[(d >> 16) & 0xFFFF] * self.ld(e(b, 0))
+            g = self.ld(e(a, 16))
+            self.st(e(a, 0), cn * self.ld(e(b, 16)))
+            self.st(e(a, 16), g + cn * ((self.ld(e(c, 0)) * self.ld(e(c, 16))) * self.ld(e(d, 0))))
+            self.st(e(b, 0), cn)
 
     def adj_gk_second(self):
         w, counts = self.gk2
+        e = lambda v, sh: ((v >> sh) & 0xFFFF) * 512
+        L = lambda v, sh: self.ld(e(v, sh))
         p = 0
         for nf, n in zip((1, 2, 3), counts):
+            assert n % 8 == 0
             for _ in range(n):
-                g, nu, lo, hi, x1, x2, x3, d1, d2, d3 = w[p:p + 10]
-                L = self.ld
+                a, b, c, d = w[p:p + 4]
                 if nf == 1:
-                    t = L(d1)
+                    t = L(c, 16)
                 elif nf == 2:
-                    t = (L(x2) * L(d1)) + (L(x1) * L(d2))
+                    t = (L(b, 16) * L(c, 16)) + (L(b, 0) * L(d, 0))
                 else:
-                    t = (((L(x2) * L(x3)) * L(d1)) + ((L(x1) * L(x3)) * L(d2))) + ((L(x1) * L(x2)) * L(d3))
-                self.st(g, self.ld(g) + (L(nu) * _coef(lo, hi)) * t)
-                p += 12
+                    t = (((L(b, 16) * L(c, 0)) * L(c, 16)) + ((L(b, 0) * L(c, 0)) * L(d, 0))) \
+                        + ((L(b, 0) * L(b, 16)) * L(d, 16))
+                self.st(e(a, 0), L(a, 0) + L(a, 16) * t)
+                p += 4
         assert p == len(w)
 
     def stream_term(self, word, v, acc):
@@ -303,8 +310,7 @@ class TableMachine:
     def adj_mono_gather(self, second, point):
         """The kernel's form: W, then the k records, then the gather sums."""
         if not second:
-            self.products("adj_w")
-            self.adj_gk_first(point)
+            self.adj_wk(point)
         else:
             self.adj_gk_second()
         self.adj_lp(second, point)
@@ -581,7 +587,8 @@ def test_adjoint_gather_equals_scatter(name):
         m.fun(False, m.o("FCN0"))
         assert not m.jac_lu(np.full(lanes, 1e3)).any()
         m.adj_step(np.full(lanes, 0.02))
-        out.append(m.ws[E["LAM"]:E["ADJ_TOTAL"]].copy())
+        keep = np.r_[E["LAM"]:E["NU"], E["GK"]:E["ADJ_TOTAL"]]   # NU is scratch here
+        out.append(m.ws[keep].copy())
     a, b = out
     assert np.isfinite(a).all()
     assert np.array_equal(a.view(np.int64), b.view(np.int64))
