@@ -30,8 +30,9 @@ from . import inputs as _inputs
 from . import lib as _lib
 from .solver import Batch, Result, _check
 
-DEFAULT_STEPS_PER_CELL = 128
+DEFAULT_STEPS_PER_CELL = 96
 STATE_BUDGET_BYTES = 32e9   # checkpoint states kept in HBM before thinning them out
+STAGE_BUDGET_BYTES = 96e9   # stage vectors kept when every step is a checkpoint
 
 
 @dataclass
@@ -41,6 +42,7 @@ class Records:
     yslot: torch.Tensor  # [cap] int32, checkpoint state slot or -1
     head: torch.Tensor   # [nCell] int32, last record of each cell
     Y: torch.Tensor      # [ycap, NVAR] checkpoint states
+    K: torch.Tensor | None  # [ycap, 4, NVAR] stage vectors of the checkpoint steps
     every: int           # checkpoint interval
     used: int
     yused: int
@@ -52,12 +54,15 @@ def _p(t):
 
 
 def record_solve(b: Batch, mech: str = "fullchem", every: int = 0, cap: int = 0,
-                 state_budget: float = STATE_BUDGET_BYTES) -> tuple[Result, Records]:
+                 state_budget: float = STATE_BUDGET_BYTES,
+                 stage_budget: float = STAGE_BUDGET_BYTES) -> tuple[Result, Records]:
     """Forward solve of the batch (persistent kernel) recording every accepted
     step, with the start state of every `every`-th step of each cell as a
     checkpoint.  every = 0 picks the smallest interval whose states fit
     `state_budget` bytes; pools that turn out too small are re-sized to the
-    exact need and the solve is repeated."""
+    exact need and the solve is repeated.  When every step is a checkpoint
+    and their stage vectors fit `stage_budget` bytes, those are kept too, so
+    that the sweep recomputes only each step's Jacobian and factors."""
     lib = _lib.load(mech)
     nvar = _lib.mech_info(lib)[0]
     _check(b, _lib.mech_info(lib))
@@ -75,6 +80,9 @@ def record_solve(b: Batch, mech: str = "fullchem", every: int = 0, cap: int = 0,
         yslot = torch.empty((cap,), dtype=torch.int32, device=dev)
         head = torch.empty((n,), dtype=torch.int32, device=dev)
         recY = torch.empty((ycap, nvar), dtype=torch.float64, device=dev)
+        recK = None
+        if ev == 1 and ycap * 4 * state_bytes <= stage_budget:
+            recK = torch.empty((ycap, 4, nvar), dtype=torch.float64, device=dev)
         used, yused = ctypes.c_long(0), ctypes.c_long(0)
         tk = ctypes.c_double(0.0)
         stream = torch.cuda.current_stream(dev).cuda_stream
@@ -82,9 +90,10 @@ def record_solve(b: Batch, mech: str = "fullchem", every: int = 0, cap: int = 0,
             n, float(b.Tstart), float(b.Tend), _p(out.C), _p(b.RCONST), _p(b.ATOL), _p(b.RTOL),
             _p(b.ICNTRL), _p(b.RCNTRL), _p(b.Perm), _p(out.ISTAT), _p(out.RSTAT), _p(out.IERR),
             _p(recH), _p(prev), _p(yslot), _p(head), cap, _p(recY), ycap, ev,
+            None if recK is None else _p(recK),
             ctypes.byref(used), ctypes.byref(yused), ctypes.c_void_p(stream), ctypes.byref(tk))
         if rc == -4:
-            del recH, prev, yslot, recY
+            del recH, prev, yslot, recY, recK
             cap = max(cap, int(used.value))
             if not every:
                 ev = max(ev, -(-int(used.value) * state_bytes // int(state_budget)))
@@ -92,8 +101,8 @@ def record_solve(b: Batch, mech: str = "fullchem", every: int = 0, cap: int = 0,
         if rc != 0:
             raise RuntimeError(f"gc_amd_record_batch_device returned {rc}: {_lib.last_error(lib)}")
         out.t_kernel = tk.value
-        return out, Records(recH, prev, yslot, head, recY, ev, int(used.value), int(yused.value),
-                            tk.value)
+        return out, Records(recH, prev, yslot, head, recY, recK, ev, int(used.value),
+                            int(yused.value), tk.value)
     raise RuntimeError("record pool sizing failed")
 
 
@@ -115,7 +124,8 @@ def adjoint_sweep(b: Batch, rec: Records, lamC: torch.Tensor, mech: str = "fullc
     rc = lib.gc_amd_adjoint_batch_device(
         n, float(b.Tstart), float(b.Tend), _p(b.C), _p(b.RCONST), _p(b.ICNTRL), _p(perm),
         _p(rec.H), _p(rec.prev), _p(rec.yslot), _p(rec.head), int(rec.H.shape[0]), _p(rec.Y),
-        int(rec.Y.shape[0]), int(rec.every), _p(lamC), _p(gC), _p(gR), _p(ierr),
+        int(rec.Y.shape[0]), int(rec.every), None if rec.K is None else _p(rec.K),
+        _p(lamC), _p(gC), _p(gR), _p(ierr),
         ctypes.c_void_p(stream), ctypes.byref(tk))
     if rc != 0:
         raise RuntimeError(f"gc_amd_adjoint_batch_device returned {rc}: {_lib.last_error(lib)}")

@@ -431,8 +431,8 @@ int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double* C,
                                const int* ICNTRL, const double* RCNTRL, const int* Perm,
                                int* ISTAT, double* RSTAT, int* IERR, double* recH,
                                int* recPrev, int* recYslot, int* head, long cap, double* recY,
-                               long ycap, int every, long* used, long* yused, void* stream,
-                               double* tKernel) {
+                               long ycap, int every, double* recK, long* used, long* yused,
+                               void* stream, double* tKernel) {
   if (nCell < 0 || cap < 0 || cap > 0x7fffffffL || ycap < 0 || ycap > 0x7fffffffL ||
       every < 1 || every > 4096) {
     set_error("gc_amd_record_batch_device: bad argument");
@@ -442,7 +442,7 @@ int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double* C,
   DevCtx* cp = enter(stream, &rc);
   if (!cp) return rc;
   GcBatchArgs b{nCell, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR};
-  GcRecord rec{recH, recPrev, recYslot, head, recY, nullptr, nullptr,
+  GcRecord rec{recH, recPrev, recYslot, head, recY, recK, nullptr, nullptr,
                (unsigned)cap, (unsigned)ycap, every};
   rc = record_solve(*cp, b, rec, used, yused, cp->stream, tKernel);
   cp->mu.unlock();
@@ -453,8 +453,8 @@ int gc_amd_adjoint_batch_device(int nCell, double Tstart, double Tend, const dou
                                 const double* RCONST, const int* ICNTRL, const int* Perm,
                                 const double* recH, const int* recPrev, const int* recYslot,
                                 const int* head, long cap, const double* recY, long ycap,
-                                int every, const double* lamC, double* gC, double* gRCONST,
-                                int* IERR, void* stream, double* tKernel) {
+                                int every, const double* recK, const double* lamC, double* gC,
+                                double* gRCONST, int* IERR, void* stream, double* tKernel) {
   if (nCell < 0 || cap < 0 || cap > 0x7fffffffL || ycap < 0 || ycap > 0x7fffffffL ||
       every < 1 || every > 64) {
     set_error("gc_amd_adjoint_batch_device: bad argument");
@@ -463,7 +463,7 @@ int gc_amd_adjoint_batch_device(int nCell, double Tstart, double Tend, const dou
   int rc = 0;
   DevCtx* cp = enter(stream, &rc);
   if (!cp) return rc;
-  GcAdjArgs a{nCell, Tstart, Tend, C, RCONST, ICNTRL, Perm, recY, recH, recPrev, recYslot,
+  GcAdjArgs a{nCell, Tstart, Tend, C, RCONST, ICNTRL, Perm, recY, recK, recH, recPrev, recYslot,
               head, (unsigned)cap, (unsigned)ycap, every, 0, lamC, gC, gRCONST, IERR};
   rc = adjoint_solve(*cp, a, cp->stream, tKernel);
   cp->mu.unlock();

@@ -790,6 +790,11 @@ __device__ __forceinline__ void record_step(const Blk& b, const GcRecord& rec, b
   if (ck && ys < rec.ycap) {
     double* y = rec.Y + (size_t)ys * NVAR;
     vpass<NVAR>([&](int k) { return b.ld(O_Y + k * 512); }, [&](int k, double v) { y[k] = v; });
+    if (rec.K) {  // the accepted step's stage vectors K_1..K_4 (contiguous elements)
+      double* kk = rec.K + (size_t)ys * 4 * NVAR;
+      vpass<4 * NVAR>([&](int k) { return b.ld(O_K + k * 512); },
+                      [&](int k, double v) { kk[k] = v; });
+    }
     yslot = (int)ys;
   }
   rec.H[slot] = H;
@@ -1297,17 +1302,25 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
           [&](int k, double v) { b.st(O_Y + k * 512, v); });
     }
     if (ierr != 1) break;
-    // the adjoint through the segment's steps, last first
+    // the adjoint through the segment's steps, last first; the checkpoint
+    // step's stage vectors come from the recording when it kept them
     for (int q = len - 1; q >= 0; --q) {
       const double H = b.ld(O_SEGH + (len - 1 - q) * 512);
       vpass<NVAR>([&](int i) { return b.ld(O_SEG + (q * NVAR + i) * 512); },
                   [&](int i, double v) { b.st(O_Y + i * 512, v); });
-      fun(b, false, O_FCN0);
+      const bool kept = q == 0 && a.recK != nullptr;
+      if (kept) {
+        const double* kk = a.recK + (size_t)ys * 4 * NVAR;
+        vpass<4 * NVAR>([&](int k) { return kk[k]; },
+                        [&](int k, double v) { b.st(O_K + k * 512, v); });
+      } else {
+        fun(b, false, O_FCN0);
+      }
       if (jac_lu(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
         ierr = -8;
         break;
       }
-      stages(b, true, Dir, H, ist);
+      if (!kept) stages(b, true, Dir, H, ist);
       adj_step(b, Dir, H, clamp);
     }
   }

@@ -43,6 +43,7 @@ struct GcRecord {
   int* yslot;         // [cap], state-pool slot of a checkpoint record, else -1
   int* head;          // [nCell] (caller's cell index): last record, -1 if none
   double* Y;          // [ycap][NVAR]
+  double* K;          // [ycap][4][NVAR] stage vectors of the checkpoint steps, or null
   unsigned* count;    // records taken; > cap means the pool overflowed
   unsigned* ycount;   // states taken; > ycap means the state pool overflowed
   unsigned cap, ycap;
@@ -58,6 +59,7 @@ struct GcAdjArgs {
   const int* ICNTRL;     // (20, nCell)
   const int* Perm;       // (nCell), 1-based launch order
   const double* recY;
+  const double* recK;    // stage vectors of the checkpoint steps, or null: recomputed
   const double* recH;
   const int* recPrev;
   const int* recYslot;

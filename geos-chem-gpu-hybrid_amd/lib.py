@@ -29,10 +29,10 @@ SIGNATURES = {
     "gc_gpu_selftest": (None, [IP, IP, DP]),
     "gc_amd_integrate_batch_device": (I, [I, I, I, D, D, P, P, P, P, P, P, P, P, P, P, P, DP]),
     "gc_amd_record_batch_device": (I, [I, D, D] + [P] * 14 + [ctypes.c_long, P, ctypes.c_long, I,
-                                       ctypes.POINTER(ctypes.c_long),
+                                       P, ctypes.POINTER(ctypes.c_long),
                                        ctypes.POINTER(ctypes.c_long), P, DP]),
     "gc_amd_adjoint_batch_device": (I, [I, D, D] + [P] * 8 + [ctypes.c_long, P, ctypes.c_long, I,
-                                        P, P, P, P, P, DP]),
+                                        P, P, P, P, P, P, DP]),
     "gc_amd_set_tuning": (I, [I, I, I, I, I]),
     "gc_amd_get_tuning": (None, [IP, IP, IP, IP, IP]),
     "gc_amd_mech_info": (None, [IP, IP, IP, IP]),

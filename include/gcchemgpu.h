@@ -124,10 +124,13 @@ void gc_amd_get_tuning(int *block, int *blocks_per_cu, int *cells_per_wave,
  * (its 1st, (every+1)-th, ...) a checkpoint slot of the state pool, whose
  * recY[slot*NVAR .. +NVAR) is the state at the start of the step, else -1.
  * head[c] (c = 0-based column of the caller's arrays) is the cell's last
- * record or -1.  The pools hold `cap` records and `ycap` states; *used and
- * *yused receive what the solve needed.  Returns 0, -4 if a pool was too small
- * (the solve results are valid, the records are not: call again with pools of
- * at least *used / *yused), or the codes of gc_amd_integrate_batch_device.
+ * record or -1.  recK, when not null, holds ycap x 4 x NVAR doubles: a
+ * checkpoint step's stage vectors K_1..K_4 go to recK[slot*4*NVAR ..), so
+ * that the sweep need not recompute them.  The pools hold `cap` records and
+ * `ycap` states; *used and *yused receive what the solve needed.  Returns 0,
+ * -4 if a pool was too small (the solve results are valid, the records are
+ * not: call again with pools of at least *used / *yused), or the codes of
+ * gc_amd_integrate_batch_device.
  * Blocking; ordered after `stream`. */
 int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double *C,
                                const double *RCONST, const double *ATOL,
@@ -135,8 +138,8 @@ int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double *C,
                                const double *RCNTRL, const int *Perm, int *ISTAT,
                                double *RSTAT, int *IERR, double *recH, int *recPrev,
                                int *recYslot, int *head, long cap, double *recY,
-                               long ycap, int every, long *used, long *yused,
-                               void *stream, double *tKernel);
+                               long ycap, int every, double *recK, long *used,
+                               long *yused, void *stream, double *tKernel);
 
 /* Reverse sweep of the discrete adjoint of the recorded accepted Rodas3 steps
  * (step sizes held fixed).  Given lamC(NSPEC,nCell) = dL/dC at Tend, writes
@@ -145,8 +148,9 @@ int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double *C,
  * gc_amd_record_batch_device call (same pools and `every`, 1..64).  Each
  * segment of a cell's steps is replayed forward from its checkpoint (bit for
  * bit the recorded trajectory), then each step's factors and stages are
- * recomputed and the adjoint is swept through it with transposed solves on
- * the same factors (KppSolveTR order).  C supplies the fixed species;
+ * recomputed (a checkpoint step's stages are taken from recK when it is not
+ * null: the recK of the recording) and the adjoint is swept through it with
+ * transposed solves on the same factors (KppSolveTR order).  C supplies the fixed species;
  * ICNTRL(1) must be non-zero (autonomous, as GEOS-Chem sets it) and
  * ICNTRL(16) = 1 (clamp) is honoured.  IERR(c) = 1, -3 for a non-autonomous
  * cell (gradients 0), -8 if a recorded step's matrix is singular, -5 if the
@@ -158,8 +162,9 @@ int gc_amd_adjoint_batch_device(int nCell, double Tstart, double Tend,
                                 const double *recH, const int *recPrev,
                                 const int *recYslot, const int *head, long cap,
                                 const double *recY, long ycap, int every,
-                                const double *lamC, double *gC, double *gRCONST,
-                                int *IERR, void *stream, double *tKernel);
+                                const double *recK, const double *lamC, double *gC,
+                                double *gRCONST, int *IERR, void *stream,
+                                double *tKernel);
 
 /* Dimensions of the mechanism compiled into this library. */
 void gc_amd_mech_info(int *nvar, int *nfix, int *nreact, int *lu_nonzero);

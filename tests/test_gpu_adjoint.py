@@ -166,6 +166,23 @@ def test_sparse_checkpoints_identical(mech, adj_case, adj_grads, every):
     assert np.array_equal(r2.cpu().numpy().view(np.int64), gR.view(np.int64))
 
 
+def test_kept_stage_vectors_identical(mech, adj_case, adj_grads):
+    """The recording keeps the checkpoint steps' stage vectors when every step
+    is a checkpoint; recomputing them instead gives the same gradients bit
+    for bit."""
+    d, b, out, rec = adj_case
+    lam, gC, gR = adj_grads
+    assert rec.every == 1 and rec.K is not None
+    _say("record + adjoint, every step a checkpoint, stage vectors recomputed, 1024 cells")
+    out2, rec2 = adjoint.record_solve(b, every=1, stage_budget=0)
+    assert rec2.K is None and torch.equal(out2.C, out.C)
+    g2, r2, ierr, _ = adjoint.adjoint_sweep(b, rec2, torch.as_tensor(lam, device="cuda"))
+    torch.cuda.synchronize()
+    assert bool((ierr == 1).all())
+    assert np.array_equal(g2.cpu().numpy().view(np.int64), gC.view(np.int64))
+    assert np.array_equal(r2.cpu().numpy().view(np.int64), gR.view(np.int64))
+
+
 def test_autograd_function(mech, adj_case, adj_grads):
     d, b, out, rec = adj_case
     lam, gC, gR = adj_grads
