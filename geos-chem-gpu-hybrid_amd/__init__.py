@@ -8,9 +8,12 @@ mirror used by the standalone driver, the tests and the gradient path:
 * ``mechanism``  mechanism IR loading and counts (flops, bytes)
 * ``lib``        ctypes binding of the C ABI (fails loudly without the library)
 * ``solver``     device-resident batch solve on torch tensors
+* ``adjoint``    discrete adjoint (recording solve, reverse sweep, autograd)
 * ``inputs``     seeded synthetic cells for the benchmark configurations
-* ``standalone`` batch driver (the reference's kpp_batch modes)
+* ``synthetic``  mechanism IRs from reaction lists (the synthetic stress mechanisms)
+* ``tuning``     stored launch parameters and measured traffic
 * ``autotune``   launch-parameter sweep of the persistent kernel
+* ``build``      in-tree hipcc build of the library (``codegen/`` tables)
 
 The directory name is not a Python identifier; it is imported as
 ``gcchem_amd`` (see ``load_package`` in the repository's entry points).

@@ -307,6 +307,7 @@ int env_chunk(int nCell) {
     for (const char* p = cs; *p; ++p) {
       if (*p < '0' || *p > '9') { ok = false; break; }
       val = val * 10 + (*p - '0');
+      if (val > 1 << 30) val = 1 << 30;  // any larger chunk means one launch anyway
     }
     if (ok && val > 0) v = val;
   }
