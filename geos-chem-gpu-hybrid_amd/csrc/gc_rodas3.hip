@@ -1005,12 +1005,23 @@ __device__ __forceinline__ void stream_term(const Blk& b, int word, double v, do
 }
 
 // nu(r) = sum_i S(i,r) mu(i) for mu in TMP, S the net stoichiometry: one
-// stream of terms {TMP(i), NU(r) | flags, coef} over all reactions, eight
-// loads in flight per batch.
+// stream of terms {TMP(i), NU(r) | flags, coef} over all reactions, sixteen
+// loads in flight per batch (the stream is a multiple of eight terms).
 __device__ void adj_nu(const Blk& b) {
   ctab_t* t = tab(gc_adj_nu);
   double acc = 0.0;
-  for (int q = 0; q < MECH_NT_NU; q += 8, t += 32) {
+  int q = 0;
+  for (; q + 16 <= MECH_NT_NU; q += 16, t += 64) {
+    i4 tt[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) tt[r] = ld4(t + 4 * r);
+    double x[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[r] = b.ld(tt[r].x);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) stream_term(b, tt[r].y, dbl(tt[r].z, tt[r].w) * x[r], acc);
+  }
+  for (; q < MECH_NT_NU; q += 8, t += 32) {
     i4 tt[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) tt[r] = ld4(t + 4 * r);
@@ -1075,11 +1086,30 @@ __device__ void adj_gk2(const Blk& b) {
 
 // Gather sums: one stream of terms {W | a<<16, b | c<<16, d, dst | flags}
 // (W..d element indices) over all targets, each term W*((a*b)+(c*d)); the
-// first term of a target is its initial value.  Eight terms per batch.
+// first term of a target is its initial value.  Sixteen terms per batch (the
+// stream is a multiple of eight terms).
 __device__ void adj_lp(const Blk& b, const int* table, int nterm) {
   ctab_t* t = tab(table);
   double acc = 0.0;
-  for (int q = 0; q < nterm; q += 8, t += 32) {
+  int q = 0;
+  for (; q + 16 <= nterm; q += 16, t += 64) {
+    i4 tt[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) tt[r] = ld4(t + 4 * r);
+    double w[16], xa[16], xb[16], xc[16], xd[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      w[r] = b.ld(eoff(tt[r].x, 0));
+      xa[r] = b.ld(eoff(tt[r].x, 16));
+      xb[r] = b.ld(eoff(tt[r].y, 0));
+      xc[r] = b.ld(eoff(tt[r].y, 16));
+      xd[r] = b.ld(eoff(tt[r].z, 0));
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      stream_term(b, tt[r].w, w[r] * ((xa[r] * xb[r]) + (xc[r] * xd[r])), acc);
+  }
+  for (; q < nterm; q += 8, t += 32) {
     i4 tt[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) tt[r] = ld4(t + 4 * r);
