@@ -150,14 +150,32 @@ def plan_membench(force: bool = False):
     return []
 
 
+LINKTEST = LIB / "gc_link_test"
+
+
+def build_linktest(force: bool = False) -> Path:
+    """The link / OpenMP-coexistence check (csrc/gc_link_test.c): plain C with
+    -fopenmp, linked against the drop-in libgcchemgpu.so found next to it."""
+    src = CSRC / "gc_link_test.c"
+    drop_in = LIB / "libgcchemgpu.so"
+    if force or _stale(LINKTEST, [src, drop_in]):
+        subprocess.run(["gcc", "-O2", "-fopenmp", f"-I{PKG.parent / 'include'}", str(src),
+                        f"-L{LIB}", "-lgcchemgpu", "-Wl,-rpath,$ORIGIN", "-o", str(LINKTEST)],
+                       check=True)
+    return LINKTEST
+
+
 def build_all(force: bool = False, mechs=None) -> list[Path]:
-    """Build every mechanism library and the calibration program; all compile
-    jobs share one pool."""
+    """Build every mechanism library, the calibration program and the link
+    check; all compile jobs share one pool."""
     plans = [plan_mechanism(m, force) for m in (mechs or available_mechanisms())]
     todo = [job for jobs, _ in plans for job in jobs] + plan_membench(force)
     if todo:
         _run_parallel(todo, max(1, min(len(todo), os.cpu_count() or 1)))
-    return [link() for _, link in plans]
+    libs = [link() for _, link in plans]
+    if (LIB / "libgcchemgpu.so").exists():
+        build_linktest(force)
+    return libs
 
 
 if __name__ == "__main__":

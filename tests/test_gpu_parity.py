@@ -88,6 +88,28 @@ def test_library_loaded_before_torch():
     assert "solved" in r.stdout
 
 
+def test_openmp_threads_around_the_library(tmp_path, mech):
+    """A C program built with -fopenmp (csrc/gc_link_test.c) links the drop-in
+    library, runs the self test, and solves one batch through
+    gc_gpu_integrate_batch and then from 4 OpenMP threads at once through
+    gc_gpu_integrate_batch_dev: every thread's results equal the single
+    call's bit for bit."""
+    import subprocess
+    d = inputs.config0(mech, n=256, seed=5)
+    n = d["C"].shape[0]
+    (tmp_path / "meta.txt").write_text(
+        f"{n} {mech.nspec} {mech.nreact} {mech.nvar} {float(d['Tstart'])!r} {float(d['Tend'])!r}\n")
+    for k, dt in (("C", np.float64), ("RCONST", np.float64), ("ATOL", np.float64),
+                  ("RTOL", np.float64), ("ICNTRL", np.int32), ("RCNTRL", np.float64)):
+        np.ascontiguousarray(d[k], dtype=dt).tofile(tmp_path / f"{k}.bin")
+    print("case: gc_link_test, 256 cells, 4 OpenMP threads", flush=True)
+    r = subprocess.run([str(L.LIB_DIR / "gc_link_test"), str(tmp_path), "4"],
+                       capture_output=True, text=True, timeout=240)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("OK")
+
+
 def test_selftest_and_info():
     lib = L.load()
     nd = ctypes.c_int()

@@ -77,3 +77,22 @@ def test_drop_in_library_name():
     for n in ("gc_gpu_integrate_batch", "gc_gpu_integrate_batch_dev", "gc_gpu_info",
               "gc_gpu_set_device", "gc_gpu_selftest"):
         assert hasattr(so, n)
+
+
+def test_link_hygiene():
+    """The drop-in library coexists with an OpenMP host program: it exports no
+    OpenMP / OpenACC runtime symbols, and a C program built with -fopenmp
+    links it by its drop-in name (the link check built next to it)."""
+    import subprocess
+    from gcchem_amd import lib as L
+    so = L.LIB_DIR / "libgcchemgpu.so"
+    out = subprocess.run(["nm", "-D", "--defined-only", str(so)], capture_output=True,
+                         text=True, check=True).stdout
+    names = [line.split()[-1] for line in out.splitlines() if line.strip()]
+    assert not [n for n in names if n.startswith(("omp_", "acc_", "GOMP_", "__kmp", "kmp_"))]
+    assert {"gc_gpu_integrate_batch", "gc_gpu_integrate_batch_dev"} <= set(names)
+    exe = L.LIB_DIR / "gc_link_test"
+    assert exe.exists(), "build() builds the link check"
+    needed = subprocess.run(["ldd", str(exe)], capture_output=True, text=True, check=True).stdout
+    assert "libgcchemgpu.so" in needed and "libgomp" in needed
+
