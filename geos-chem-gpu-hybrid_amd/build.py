@@ -8,7 +8,8 @@ For each mechanism IR under ``mechanisms/``:
 2. ``csrc/gc_rodas3.hip`` is compiled once per kernel variant
    (``-DGC_VARIANT=1``: 2 waves per SIMD, 32-entry LU row segments in LDS;
    ``-DGC_VARIANT=2``: 4 waves per SIMD, 16-entry segments) and
-   ``csrc/gc_api.hip`` once, in parallel;
+   ``csrc/gc_api.hip`` once, and the host-only ``csrc/gc_hybrid.cpp`` (cost
+   order and hybrid CPU/GPU split) with g++, in parallel;
 3. the objects are linked into ``lib/libgcchemgpu_<mech>.so``; the fullchem
    library is also installed as ``lib/libgcchemgpu.so``, the drop-in name the
    model links (src/GCClassic/CMakeLists.txt:204-228 links ``GC_GPU_LIB``).
@@ -115,10 +116,18 @@ def plan_mechanism(mech: str, force: bool = False):
         todo.append(([HIPCC, *HIP_FLAGS, f"-I{CSRC}", "-c", str(CSRC / "gc_api.hip"),
                       "-o", str(obj)], f"{mech}/api.o"))
 
+    obj = bdir / "hybrid.o"
+    objs.append(obj)
+    deps = [CSRC / "gc_hybrid.cpp", PKG.parent / "include" / "gcchemgpu.h"]
+    if force or _stale(obj, deps):
+        todo.append((["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall",
+                      f"-I{PKG.parent / 'include'}", "-c", str(CSRC / "gc_hybrid.cpp"),
+                      "-o", str(obj)], f"{mech}/hybrid.o"))
+
     def link_one(out: Path):
         if force or _stale(out, objs):
             tmp = out.with_suffix(".so.tmp")
-            subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC",
+            subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread",
                             f"-Wl,-soname,{out.name}", "-o", str(tmp), *map(str, objs)],
                            check=True)
             os.replace(tmp, out)

@@ -20,6 +20,29 @@ D = ctypes.c_double
 IP = ctypes.POINTER(ctypes.c_int)
 DP = ctypes.POINTER(ctypes.c_double)
 
+MAX_DEV = 16
+
+
+class GcAmdHybrid(ctypes.Structure):
+    """struct GcAmdHybrid of include/gcchemgpu.h (the split's state across steps)."""
+    _fields_ = [("work_frac", D), ("fixed_frac", I), ("ndev", I), ("rate_gpu", D),
+                ("rate_cpu", D), ("dev_rate", D * MAX_DEV), ("n_gpu", I), ("n_dev_use", I),
+                ("dev_lo", I * MAX_DEV), ("dev_hi", I * MAX_DEV), ("t_gpu", D),
+                ("t_kernel", D), ("t_cpu", D), ("t_region", D)]
+
+    @classmethod
+    def initial(cls, work_frac=0.5, fixed_frac=False, ndev=1):
+        st = cls()
+        st.work_frac, st.fixed_frac, st.ndev = work_frac, int(fixed_frac), ndev
+        st.rate_gpu = st.rate_cpu = -1.0
+        for d in range(MAX_DEV):
+            st.dev_rate[d] = -1.0
+        return st
+
+
+# gc_amd_cpu_tail_fn: void (*)(int first, int last, void *user)
+CPU_TAIL_FN = ctypes.CFUNCTYPE(None, I, I, P)
+
 # every symbol of include/gcchemgpu.h with its signature (restype, argtypes)
 SIGNATURES = {
     "gc_gpu_integrate_batch": (None, [I, D, D, P, P, P, P, P, P, P, P, P, P, DP, DP]),
@@ -33,6 +56,11 @@ SIGNATURES = {
                                        ctypes.POINTER(ctypes.c_long), P, DP]),
     "gc_amd_adjoint_batch_device": (I, [I, D, D] + [P] * 8 + [ctypes.c_long, P, ctypes.c_long, I,
                                         P, P, P, P, P, P, DP]),
+    "gc_amd_cost_slots": (I, [I, P, I, P, P]),
+    "gc_amd_hybrid_plan": (I, [I, P, I, DP, D, D, I, P, P, IP, IP, P, P]),
+    "gc_amd_hybrid_update": (None, [I, I, P, I, P, P, P, D, D, DP, DP, P]),
+    "gc_amd_hybrid_step": (I, [I, D, D] + [P] * 10 + [ctypes.POINTER(GcAmdHybrid),
+                                                       CPU_TAIL_FN, P]),
     "gc_amd_set_tuning": (I, [I, I, I, I, I]),
     "gc_amd_get_tuning": (None, [IP, IP, IP, IP, IP]),
     "gc_amd_mech_info": (None, [IP, IP, IP, IP]),

@@ -11,8 +11,10 @@
  *
  * The gc_amd_* symbols are additions of this library (no counterpart in the
  * reference): a device-resident form of the batch solve for callers that keep
- * the batch in HBM (the PyTorch binding, benchmarks), launch tuning, and a
- * query of the compiled mechanism's dimensions.
+ * the batch in HBM (the PyTorch binding, benchmarks), launch tuning, the
+ * discrete adjoint, the cost order and hybrid CPU/GPU split that the
+ * reference keeps in its caller, and a query of the compiled mechanism's
+ * dimensions.
  */
 #ifndef GCCHEMGPU_H
 #define GCCHEMGPU_H
@@ -165,6 +167,99 @@ int gc_amd_adjoint_batch_device(int nCell, double Tstart, double Tend,
                                 const double *recK, const double *lamC, double *gC,
                                 double *gRCONST, int *IERR, void *stream,
                                 double *tKernel);
+
+/* ---- cost order and the hybrid CPU/GPU split ------------------------------
+ * The reference runs these steps inside its caller
+ * (src/geos-chem/GeosCore/fullchem_mod.F90, batch mode 3); here they are
+ * library entry points with the same arithmetic.  Slots and cells are
+ * 1-based, as in the caller. */
+
+#define GC_AMD_MAX_DEV 16
+
+/* Stable counting sort of a batch's cells by the previous chemistry step's
+ * internal step count, most expensive first (the slot assignment of
+ * fullchem_mod.F90:905-981).  Cell c (1-based, in the caller's stock L,J,I
+ * order) gets slot slotOf[c-1]; cellAt[s-1] = c is the inverse map.  sort = 0
+ * keeps the stock order (GC_BATCH_SORT=0).  Returns 0, -1 for nCell < 0, a
+ * null array or a negative step count, -2 if host memory ran out. */
+int gc_amd_cost_slots(int nCell, const int *prevNsteps, int sort, int *slotOf,
+                      int *cellAt);
+
+/* Work model and split of a cost-ordered batch (fullchem_mod.F90:997-1061).
+ *   slotNsteps(nCell)   in: previous internal step count of the cell in each slot
+ *   fixedFrac           by value: nonzero keeps *workFrac (GC_GPU_FRAC_FIXED=1)
+ *   workFrac            inout: GPU share of the work (initially 0.5 or
+ *                       GC_GPU_FRAC); with both rates > 0 and fixedFrac = 0 it
+ *                       becomes rateGpu/(rateGpu+rateCpu) clipped to [0.01,0.99]
+ *   rateGpu, rateCpu    by value: measured work units per second (<= 0: none yet)
+ *   nDev                by value: devices to drive (1..GC_AMD_MAX_DEV)
+ *   devRate(nDev)       in (may be NULL): per-device rates; used as slice
+ *                       weights once every used device has one, else equal
+ *   work(0:nCell)       out: prefix sums of max(step count, 1)
+ *   nGpu                out: slots 1..nGpu go to the devices, the rest to the CPU
+ *   nDevUse             out: devices used (min(nDev, max(nGpu, 1)))
+ *   devLo, devHi(nDev)  out: device d solves slots devLo[d]..devHi[d]
+ *                       (contiguous, work-balanced; 1 and 0 when unused)
+ * Returns 0, or -1 for a bad argument. */
+int gc_amd_hybrid_plan(int nCell, const int *slotNsteps, int fixedFrac,
+                       double *workFrac, double rateGpu, double rateCpu, int nDev,
+                       const double *devRate, long long *work, int *nGpu,
+                       int *nDevUse, int *devLo, int *devHi);
+
+/* Measured-rate update after a split solve (fullchem_mod.F90:2672-2700):
+ * device d's rate from work[devHi]-work[devLo-1] over devTw[d], the GPU rate
+ * from work[nGpu] over gpuTw (the slowest device's wall time), the CPU rate
+ * from work[nCell]-work[nGpu] over tCpu; each is an exponential moving
+ * average (the first value as is, then half old, half new).  Measurements
+ * that are absent (no slots, time <= 0) leave their rate unchanged. */
+void gc_amd_hybrid_update(int nCell, int nGpu, const long long *work, int nDevUse,
+                          const int *devLo, const int *devHi, const double *devTw,
+                          double gpuTw, double tCpu, double *rateGpu,
+                          double *rateCpu, double *devRate);
+
+/* State of the hybrid split across chemistry steps (the caller's SAVE
+ * variables in the reference, fullchem_mod.F90:139-148).  Set work_frac
+ * (0.5), fixed_frac (0), ndev (1..16), rate_gpu, rate_cpu and dev_rate[]
+ * (-1: not measured) before the first step; the rest is written by
+ * gc_amd_hybrid_step. */
+typedef struct GcAmdHybrid {
+  double work_frac;
+  int fixed_frac;
+  int ndev;
+  double rate_gpu;
+  double rate_cpu;
+  double dev_rate[GC_AMD_MAX_DEV];
+  int n_gpu;                    /* out: slots 1..n_gpu went to the devices */
+  int n_dev_use;                /* out */
+  int dev_lo[GC_AMD_MAX_DEV];   /* out: device slices, 1-based inclusive */
+  int dev_hi[GC_AMD_MAX_DEV];
+  double t_gpu;                 /* out: slowest device's wall time, s */
+  double t_kernel;              /* out: kernel seconds summed over devices */
+  double t_cpu;                 /* out: wall time of the CPU tail, s */
+  double t_region;              /* out: wall time of the whole split solve, s */
+} GcAmdHybrid;
+
+/* The caller's CPU solve of slots first..last (1-based, inclusive). */
+typedef void (*gc_amd_cpu_tail_fn)(int first, int last, void *user);
+
+/* One hybrid chemistry step over a batch already in slot (cost) order: plan
+ * the split with gc_amd_hybrid_plan, solve each device's slice with
+ * gc_gpu_integrate_batch_dev from its own host thread, concurrently run
+ * cpuTail(nGpu+1, nCell, user) on the calling thread (not called when every
+ * slot went to the devices), then update the rates with
+ * gc_amd_hybrid_update.  The batch arrays have the layouts of
+ * gc_gpu_integrate_batch with nCell columns in slot order; only columns
+ * 1..nGpu are read and written.  The CPU rate is measured over cpuTail's own
+ * wall time (the reference, whose CPU threads also issue the launches,
+ * measures from the first launch to the end of its solve loop).  Returns the
+ * number of device slots with IERR < 0 (the caller retries them on the CPU,
+ * as fullchem_mod.F90:2603-2650 does), or -1 for a bad argument, -2 if host
+ * memory ran out. */
+int gc_amd_hybrid_step(int nCell, double Tstart, double Tend, double *C,
+                       const double *RCONST, const double *ATOL, const double *RTOL,
+                       const int *ICNTRL, const double *RCNTRL,
+                       const int *slotNsteps, int *ISTAT, double *RSTAT, int *IERR,
+                       GcAmdHybrid *state, gc_amd_cpu_tail_fn cpuTail, void *user);
 
 /* Dimensions of the mechanism compiled into this library. */
 void gc_amd_mech_info(int *nvar, int *nfix, int *nreact, int *lu_nonzero);

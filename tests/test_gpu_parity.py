@@ -180,6 +180,52 @@ def test_host_array_entry_points(cfg0, ref0, mech):
     assert tt.value == 0.0 and tk.value == 0.0
 
 
+def test_hybrid_step_split_and_concurrency(cfg0, ref0, mech):
+    """gc_amd_hybrid_step on the 2,048 config-0 cells in cost order (previous
+    step counts = the oracle's Nstp): the head goes to the GPU and is
+    bit-identical to the oracle, the caller's CPU solve is called once for
+    exactly the tail and runs while the GPU solves, and the next step's split
+    follows the measured rates."""
+    import time
+    lib = L.load()
+    n = cfg0["C"].shape[0]
+    prev = np.ascontiguousarray(ref0[1][:, 2], dtype=np.int32)
+    slot, cell = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    assert lib.gc_amd_cost_slots(n, p(prev), 1, p(slot), p(cell)) == 0
+    order = cell - 1
+    d = {k: np.ascontiguousarray(cfg0[k][order]) for k in ("C", "RCONST", "ICNTRL", "RCNTRL")}
+    nsteps = np.ascontiguousarray(prev[order])
+    calls = []
+
+    @L.CPU_TAIL_FN
+    def tail(first, last, user):
+        calls.append((first, last))
+        time.sleep(0.4)        # the caller's CPU solve of the tail
+
+    st = L.GcAmdHybrid.initial(work_frac=0.5, ndev=1)
+    for step in range(2):
+        C = d["C"].copy()
+        IS, RS, IE = np.zeros((n, 8), np.int32), np.zeros((n, 3)), np.zeros(n, np.int32)
+        frac_rates = (st.rate_gpu / (st.rate_gpu + st.rate_cpu)
+                      if st.rate_gpu > 0 and st.rate_cpu > 0 else 0.5)
+        print(f"case: hybrid step {step}, {n} cells", flush=True)
+        rc = lib.gc_amd_hybrid_step(n, 0.0, 600.0, p(C), p(d["RCONST"]), p(cfg0["ATOL"]),
+                                    p(cfg0["RTOL"]), p(d["ICNTRL"]), p(d["RCNTRL"]), p(nsteps),
+                                    p(IS), p(RS), p(IE), ctypes.byref(st), tail, None)
+        ng = st.n_gpu
+        assert rc == 0 and 0 < ng < n
+        assert st.work_frac == pytest.approx(min(max(frac_rates, 0.01), 0.99), rel=1e-12)
+        assert calls[-1] == (ng + 1, n) and len(calls) == step + 1
+        assert (st.n_dev_use, st.dev_lo[0], st.dev_hi[0]) == (1, 1, ng)
+        head = order[:ng]
+        _assert_bitexact((C[:ng], IS[:ng], RS[:ng], IE[:ng]),
+                         tuple(x[head] for x in ref0[:4]), mech.nvar)
+        assert np.array_equal(C[ng:], d["C"][ng:])          # the tail is the caller's
+        assert st.t_gpu > 0 and st.t_cpu >= 0.4 and st.rate_gpu > 0 and st.rate_cpu > 0
+        assert st.t_region < st.t_gpu + st.t_cpu - 0.2      # the two sides overlapped
+
+
 @pytest.mark.parametrize("kernel", ["plain", "persistent"])
 def test_permutation_and_chunks(cfg0, ref0, mech, kernel):
     """Cells are independent: any launch order and any chunking gives the same
