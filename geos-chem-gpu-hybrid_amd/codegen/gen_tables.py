@@ -50,9 +50,11 @@ Tables (int32 words):
   {slot, LU, slot, LU} (n a multiple of 2); END {0,...}.
   (gckpp_Jacobian.F90:3674-20887, gckpp_BatchIntegrator.F90:573-577,
   gckpp_LinearAlgebra.F90:46-84)
-* ``fwd<s>``/``bwd<s>`` (stage vector K_s) and ``tr1``/``tr2`` (the TMP
-  vector): solve rows, header {n, x_i, U(i,i) or 0, 0} then n/2
-  {LU, x_j, LU, x_j} with n a multiple of 4, absolute offsets.
+* ``fwd<s>`` (stage vector K_s) and ``tr1``/``tr2`` (the TMP vector): solve
+  rows, header {n, x_i, U(i,i) or 0, 0} then n/2 {LU, x_j, LU, x_j} with n a
+  multiple of 4, absolute offsets.  ``bwd<s>_<S>``: the backward rows of
+  K_s with the x_j of the last S columns read from LDS slots (the LU
+  segment's, idle during solves; see Tables.bwd_lds).
   (gckpp_LinearAlgebra.F90:645-2311, 2327-3701)
 * discrete adjoint: ``adj_nu`` (nu = S^T mu per reaction, 4-word terms
   {TMP(i), NU(r) | flags, coef}, S the net
@@ -577,6 +579,45 @@ class Tables:
                 for i in reversed(range(ir["nvar"]))]
         return self._rows(rows), len(rows)
 
+    def bwd_lds(self, vbase, stage, seg):
+        """Backward rows whose x_j of the last `seg` columns come from LDS.
+
+        Those columns carry most of the backward solve's x loads (the dense
+        trailing block, referenced by many earlier rows), and their rows are
+        the first `seg` rows solved, so each such x_i is put into LDS slot
+        i - (nvar - seg) as it is solved and read from there afterwards.  The
+        terms keep KppSolve's ascending-column order: cold columns first, then
+        the LDS suffix.  Row: {nc | nh << 16, x_i, U(i,i), lds_i or -1} +
+        nc/2 {LU, x_j, LU, x_j} + nh/2 {LU, slot_j*512, LU, slot_j*512}; nc
+        and nh are padded to multiples of 4, cold padding with (ZERO, ZERO),
+        LDS padding with (ZERO, spare slot seg), which the solve sets to 0.0.
+        """
+        ir = self.ir
+        n = ir["nvar"]
+        crow, icol, diag = ir["crow"], ir["icol"], ir["diag"]
+        h0 = n - seg
+        z = self.off("ZERO", 0)
+        x = lambda i: self.vec(vbase, stage, i)
+        out = []
+        for i in reversed(range(n)):
+            cold, hot = [], []
+            for k in range(diag[i] + 1, crow[i + 1]):
+                j = icol[k]
+                if j >= h0:
+                    hot.append((self.off("LU", k), (j - h0) * 512))
+                else:
+                    assert not hot, "U columns of a row must ascend"
+                    cold.append((self.off("LU", k), x(j)))
+            while len(cold) % 4:
+                cold.append((z, z))
+            while len(hot) % 4:
+                hot.append((z, seg * 512))
+            out += [len(cold) | len(hot) << 16, x(i), self.off("LU", diag[i]),
+                    (i - h0) * 512 if i >= h0 else -1]
+            for a, b in cold + hot:
+                out += [a, b]
+        return out, n
+
     def tr(self, vbase="TMP"):
         """KppSolveTR in place on one vector: phase 1 U^T (rows ascending,
         U(j,i) for ascending j, divide), phase 2 L^T (rows descending, L(j,i)
@@ -637,9 +678,12 @@ def generate(ir: dict) -> str:
     nf = nbw = 0
     for stage in range(4):
         fw, nf = t.fwd("K", stage)
-        bw, nbw = t.bwd("K", stage)
-        body += [_arr(f"gc_fwd{stage}", fw), _arr(f"gc_bwd{stage}", bw)]
+        body.append(_arr(f"gc_fwd{stage}", fw))
         sizes[f"fwd{stage}"] = len(fw)
+        for s in SEGMENTS:
+            bw, nbw = t.bwd_lds("K", stage, s)
+            body.append(_arr(f"gc_bwd{stage}_{s}", bw))
+        sizes[f"bwd{stage}"] = len(bw)
     tr1, n1, tr2, n2 = t.tr("TMP")
     body += [_arr("gc_tr1", tr1), _arr("gc_tr2", tr2)]
     body.append(_arr("gc_adj_nu", t.adj_nu_stream()))

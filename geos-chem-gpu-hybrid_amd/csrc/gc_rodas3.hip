@@ -514,12 +514,86 @@ __device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows) {
   }
 }
 
+// Backward rows whose x_j of the last GC_SEG columns sit in this lane's LDS
+// segment slots: header {nc | nh << 16, x_i, U(i,i), lds_i} + nc/2
+// {LU, x_j, LU, x_j} + nh/2 {LU, slot, LU, slot} (nc, nh multiples of 4):
+// x_i = x_i - LU*x_j over the cold then the LDS pairs (KppSolve's ascending
+// column order), / U(i,i); a row of that suffix also leaves x_i in its slot
+// (lds_i >= 0), before any later row reads it.  LDS padding pairs read the
+// spare slot, set to 0.0 here: x - 0.0*0.0 == x.
+__device__ __forceinline__ void solve_bwd_lds(const Blk& b, ctab_t* t, int rows) {
+  b.lds_st(GC_SEG * 512, 0.0);
+  for (int r = 0; r < rows; ++r) {
+    const i4 h = ld4(t);
+    t += 4;
+    const int nc = h.x & 0xffff, nh = h.x >> 16;
+    double x = b.ld(h.y);
+    int u = 0;
+    for (; u + 8 <= nc; u += 8, t += 16) {
+      i4 q[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) q[k] = ld4(t + 4 * k);
+      double l[8], y[8];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        l[2 * k] = b.ld(q[k].x);
+        y[2 * k] = b.ld(q[k].y);
+        l[2 * k + 1] = b.ld(q[k].z);
+        y[2 * k + 1] = b.ld(q[k].w);
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x = x - l[k] * y[k];
+    }
+    for (; u < nc; u += 4, t += 8) {
+      const i4 q01 = ld4(t), q23 = ld4(t + 4);
+      const double l0 = b.ld(q01.x), x0 = b.ld(q01.y), l1 = b.ld(q01.z), x1 = b.ld(q01.w);
+      const double l2 = b.ld(q23.x), x2 = b.ld(q23.y), l3 = b.ld(q23.z), x3 = b.ld(q23.w);
+      x = x - l0 * x0;
+      x = x - l1 * x1;
+      x = x - l2 * x2;
+      x = x - l3 * x3;
+    }
+    u = 0;
+    for (; u + 8 <= nh; u += 8, t += 16) {
+      i4 q[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) q[k] = ld4(t + 4 * k);
+      double l[8], y[8];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        l[2 * k] = b.ld(q[k].x);
+        l[2 * k + 1] = b.ld(q[k].z);
+        y[2 * k] = b.lds_ld(q[k].y);
+        y[2 * k + 1] = b.lds_ld(q[k].w);
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x = x - l[k] * y[k];
+    }
+    for (; u < nh; u += 4, t += 8) {
+      const i4 q01 = ld4(t), q23 = ld4(t + 4);
+      const double l0 = b.ld(q01.x), l1 = b.ld(q01.z), l2 = b.ld(q23.x), l3 = b.ld(q23.z);
+      const double x0 = b.lds_ld(q01.y), x1 = b.lds_ld(q01.w);
+      const double x2 = b.lds_ld(q23.y), x3 = b.lds_ld(q23.w);
+      x = x - l0 * x0;
+      x = x - l1 * x1;
+      x = x - l2 * x2;
+      x = x - l3 * x3;
+    }
+    x = x / b.ld(h.z);
+    b.st(h.y, x);
+    if (h.w >= 0) b.lds_st(h.w, x);
+  }
+}
+
 // KppSolve (gckpp_LinearAlgebra.F90:645-2311) in place on stage vector K_s.
 __device__ void solve(const Blk& b, int s) {
   const int* fw = s == 0 ? gc_fwd0 : s == 1 ? gc_fwd1 : s == 2 ? gc_fwd2 : gc_fwd3;
-  const int* bw = s == 0 ? gc_bwd0 : s == 1 ? gc_bwd1 : s == 2 ? gc_bwd2 : gc_bwd3;
+  const int* bw = s == 0   ? GC_CAT(gc_bwd0_, GC_SEG)
+                  : s == 1 ? GC_CAT(gc_bwd1_, GC_SEG)
+                  : s == 2 ? GC_CAT(gc_bwd2_, GC_SEG)
+                           : GC_CAT(gc_bwd3_, GC_SEG);
   solve_rows<false>(b, tab(fw), MECH_NFWD);
-  solve_rows<true>(b, tab(bw), MECH_NBWD);
+  solve_bwd_lds(b, tab(bw), MECH_NBWD);
 }
 
 // Options after the merge of gc_gpu_batch_api.F90:79-88 and the decode of
