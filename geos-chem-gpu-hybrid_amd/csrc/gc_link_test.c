@@ -6,7 +6,11 @@
  * test, solves a batch once through gc_gpu_integrate_batch and then the same
  * batch from several OpenMP threads at once through
  * gc_gpu_integrate_batch_dev, and checks every thread's results against the
- * single call bit for bit.
+ * single call bit for bit.  Last, one hybrid split step
+ * (gc_amd_hybrid_step): the library launches the head on its own host
+ * thread while this program's CPU-tail callback runs an OpenMP team over
+ * the tail (standing in for the caller's CPU solve, it copies the single
+ * call's answers), and the combined batch must equal the single call.
  *
  * usage: gc_link_test <dir> <threads>
  *   <dir> holds the batch as raw little-endian arrays: meta.txt ("nCell
@@ -62,6 +66,29 @@ static int out_same(const struct Out *a, const struct Out *b, int n, int nspec) 
          memcmp(a->RSTAT, b->RSTAT, sizeof(double) * (size_t)n * 3) == 0 &&
          memcmp(a->ISTAT, b->ISTAT, sizeof(int) * (size_t)n * 8) == 0 &&
          memcmp(a->IERR, b->IERR, sizeof(int) * (size_t)n) == 0;
+}
+
+/* The caller's CPU solve of slots first..last, as gc_amd_hybrid_step calls
+ * it; here every slot takes the single call's answer, from an OpenMP team. */
+struct Tail {
+  const struct Out *ref;
+  struct Out *out;
+  int nspec, threads, calls, first, last;
+};
+
+static void cpu_tail(int first, int last, void *user) {
+  struct Tail *t = (struct Tail *)user;
+  t->calls++;
+  t->first = first;
+  t->last = last;
+#pragma omp parallel for num_threads(t->threads) schedule(dynamic, 16)
+  for (int s = first; s <= last; ++s) {
+    const size_t c = (size_t)(s - 1);
+    memcpy(t->out->C + c * t->nspec, t->ref->C + c * t->nspec, sizeof(double) * t->nspec);
+    memcpy(t->out->ISTAT + c * 8, t->ref->ISTAT + c * 8, sizeof(int) * 8);
+    memcpy(t->out->RSTAT + c * 3, t->ref->RSTAT + c * 3, sizeof(double) * 3);
+    t->out->IERR[c] = t->ref->IERR[c];
+  }
 }
 
 int main(int argc, char **argv) {
@@ -130,6 +157,29 @@ int main(int argc, char **argv) {
   }
   printf("threads: %d concurrent calls, %d differ from the single call\n", threads, differ);
   fails += differ != 0;
+
+  /* hybrid split: half of the work (by the single call's step counts) to the
+   * device, the rest to the OpenMP tail, concurrently */
+  int *slotNsteps = malloc(sizeof(int) * (size_t)n);
+  for (int i = 0; i < n; ++i) slotNsteps[i] = ref.ISTAT[(size_t)i * 8 + 2];
+  GcAmdHybrid st;
+  memset(&st, 0, sizeof st);
+  st.work_frac = 0.5;
+  st.ndev = 1;
+  st.rate_gpu = st.rate_cpu = -1.0;
+  for (int d = 0; d < GC_AMD_MAX_DEV; ++d) st.dev_rate[d] = -1.0;
+  struct Out h = out_new(n, nspec, C0);
+  struct Tail tail = {&ref, &h, nspec, threads, 0, 0, 0};
+  const int nfail = gc_amd_hybrid_step(n, Tstart, Tend, h.C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL,
+                                       slotNsteps, h.ISTAT, h.RSTAT, h.IERR, &st, cpu_tail,
+                                       &tail);
+  const int same = out_same(&h, &ref, n, nspec);
+  printf("hybrid: %d slots on the device, %d in the OpenMP tail (%d call, slots %d-%d), "
+         "%d failed, %s the single call\n",
+         st.n_gpu, n - st.n_gpu, tail.calls, tail.first, tail.last, nfail,
+         same ? "same as" : "DIFFERENT from");
+  fails += !(nfail == 0 && same && st.n_gpu > 0 && st.n_gpu < n && tail.calls == 1 &&
+             tail.first == st.n_gpu + 1 && tail.last == n && st.rate_gpu > 0.0);
   printf(fails ? "FAIL\n" : "OK\n");
   return fails ? 1 : 0;
 }

@@ -93,7 +93,9 @@ def test_openmp_threads_around_the_library(tmp_path, mech):
     library, runs the self test, and solves one batch through
     gc_gpu_integrate_batch and then from 4 OpenMP threads at once through
     gc_gpu_integrate_batch_dev: every thread's results equal the single
-    call's bit for bit."""
+    call's bit for bit.  Then one gc_amd_hybrid_step whose CPU-tail callback
+    opens an OpenMP team while the library's own thread drives the device:
+    the combined batch equals the single call."""
     import subprocess
     d = inputs.config0(mech, n=256, seed=5)
     n = d["C"].shape[0]
@@ -224,6 +226,25 @@ def test_hybrid_step_split_and_concurrency(cfg0, ref0, mech):
         assert np.array_equal(C[ng:], d["C"][ng:])          # the tail is the caller's
         assert st.t_gpu > 0 and st.t_cpu >= 0.4 and st.rate_gpu > 0 and st.rate_cpu > 0
         assert st.t_region < st.t_gpu + st.t_cpu - 0.2      # the two sides overlapped
+    # a device slice that cannot run (a second device on a one-device box)
+    # comes back as IERR -99 for every one of its slots and is counted for
+    # the caller's CPU retry; the first device's slice is unaffected
+    ndev = ctypes.c_int()
+    lib.gc_gpu_info(ctypes.byref(ndev), ctypes.create_string_buffer(64), 64)
+    if ndev.value == 1:
+        st = L.GcAmdHybrid.initial(work_frac=0.5, fixed_frac=True, ndev=2)
+        C = d["C"].copy()
+        IS, RS, IE = np.zeros((n, 8), np.int32), np.zeros((n, 3)), np.zeros(n, np.int32)
+        print("case: hybrid step, 2 devices on a 1-device box", flush=True)
+        rc = lib.gc_amd_hybrid_step(n, 0.0, 600.0, p(C), p(d["RCONST"]), p(cfg0["ATOL"]),
+                                    p(cfg0["RTOL"]), p(d["ICNTRL"]), p(d["RCNTRL"]), p(nsteps),
+                                    p(IS), p(RS), p(IE), ctypes.byref(st), tail, None)
+        lo1, hi1 = st.dev_lo[1], st.dev_hi[1]
+        assert st.n_dev_use == 2 and st.dev_lo[0] == 1 and st.dev_hi[0] == lo1 - 1
+        assert rc == hi1 - lo1 + 1 > 0 and np.all(IE[lo1 - 1:hi1] == -99)
+        head = order[:lo1 - 1]
+        _assert_bitexact((C[:lo1 - 1], IS[:lo1 - 1], RS[:lo1 - 1], IE[:lo1 - 1]),
+                         tuple(x[head] for x in ref0[:4]), mech.nvar)
 
 
 @pytest.mark.parametrize("kernel", ["plain", "persistent"])
