@@ -6,7 +6,9 @@ driver binds through ISO_C_BINDING.  This Python package is the host-side
 mirror used by the standalone driver, the tests and the gradient path:
 
 * ``mechanism``  mechanism IR loading and counts (flops, bytes)
-* ``lib``        ctypes binding of the C ABI (fails loudly without the library)
+* ``lib``        ctypes binding of the C ABI (fails loudly without the library),
+                 including the cost order and hybrid CPU/GPU split entry points
+                 (``GcAmdHybrid``, ``CPU_TAIL_FN``)
 * ``solver``     device-resident batch solve on torch tensors
 * ``adjoint``    discrete adjoint (recording solve, reverse sweep, autograd)
 * ``inputs``     seeded synthetic cells for the benchmark configurations

@@ -174,6 +174,25 @@ def build_linktest(force: bool = False) -> Path:
     return LINKTEST
 
 
+FORTRAN_CALLER = LIB / "gc_fortran_caller"
+FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
+
+
+def build_fortran_caller(force: bool = False) -> Path | None:
+    """The Fortran caller check (csrc/gc_fortran_caller.F90): flang -fopenmp,
+    binding the drop-in library through ISO_C_BINDING as GEOS-Chem does.
+    Returns None when no flang is installed."""
+    src = CSRC / "gc_fortran_caller.F90"
+    drop_in = LIB / "libgcchemgpu.so"
+    if not Path(FLANG).exists():
+        return None
+    if force or _stale(FORTRAN_CALLER, [src, drop_in]):
+        subprocess.run([FLANG, "-O2", "-fopenmp", f"-J{BUILD}", str(src), f"-L{LIB}",
+                        "-lgcchemgpu", "-Wl,-rpath,$ORIGIN", "-o", str(FORTRAN_CALLER)],
+                       check=True, cwd=str(BUILD))
+    return FORTRAN_CALLER
+
+
 def build_all(force: bool = False, mechs=None) -> list[Path]:
     """Build every mechanism library, the calibration program and the link
     check; all compile jobs share one pool."""
@@ -184,6 +203,7 @@ def build_all(force: bool = False, mechs=None) -> list[Path]:
     libs = [link() for _, link in plans]
     if (LIB / "libgcchemgpu.so").exists():
         build_linktest(force)
+        build_fortran_caller(force)
     return libs
 
 

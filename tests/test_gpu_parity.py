@@ -112,6 +112,25 @@ def test_openmp_threads_around_the_library(tmp_path, mech):
     assert r.stdout.strip().endswith("OK")
 
 
+def test_fortran_caller():
+    """A Fortran program built with flang -fopenmp (csrc/gc_fortran_caller.F90)
+    binds the library through ISO_C_BINDING with the interfaces of
+    fullchem_mod.F90:153-212: the self test's known answer, the reference
+    multi-device driver's dummy chemistry converging in all 4,096 cells, four
+    OpenMP threads solving slices (slice base addresses, full identity Perm)
+    bit-identical to one call, and the hybrid plan through its Fortran
+    binding."""
+    import subprocess
+    exe = L.LIB_DIR / "gc_fortran_caller"
+    if not exe.exists():
+        pytest.skip("no Fortran compiler was found when the library was built")
+    print("case: gc_fortran_caller, 4096 cells, 4 OpenMP threads", flush=True)
+    r = subprocess.run([str(exe), "4096", "4"], capture_output=True, text=True, timeout=240)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().splitlines()[-1] == "OK"
+
+
 def test_selftest_and_info():
     lib = L.load()
     nd = ctypes.c_int()
@@ -180,6 +199,35 @@ def test_host_array_entry_points(cfg0, ref0, mech):
                                p(d["ICNTRL"]), p(d["RCNTRL"]), p(perm), p(IS), p(RS), p(IE),
                                ctypes.byref(tt), ctypes.byref(tk))
     assert tt.value == 0.0 and tk.value == 0.0
+
+
+def test_environment_selects_kernel_and_chunks(cfg0, ref0, mech, monkeypatch):
+    """The reference's run-time switches (gpulib/src/gc_gpu_batch_api.F90:
+    231-256), read at every host-array call: GC_GPU_KERNEL=plain with
+    GC_GPU_WORKSPACE=global solves in chunks of GC_GPU_CHUNK cells (clamped
+    to [1024, nCell]); a malformed chunk size falls back to the default.
+    Every setting gives the oracle's results bit for bit."""
+    lib = L.load()
+    n = cfg0["C"].shape[0]
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    perm = np.arange(1, n + 1, dtype=np.int32)
+    for env in ({"GC_GPU_KERNEL": "plain"},
+                {"GC_GPU_KERNEL": "plain", "GC_GPU_WORKSPACE": "global", "GC_GPU_CHUNK": "1000"},
+                {"GC_GPU_KERNEL": "plain", "GC_GPU_WORKSPACE": "global", "GC_GPU_CHUNK": "12x"},
+                {"GC_GPU_WORKSPACE": "global", "GC_GPU_CHUNK": "1024"}):
+        for k in ("GC_GPU_KERNEL", "GC_GPU_WORKSPACE", "GC_GPU_CHUNK"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        print(f"case: host-array call, {n} cells, {env}", flush=True)
+        C = np.ascontiguousarray(cfg0["C"].copy())
+        IS, RS, IE = np.zeros((n, 8), np.int32), np.zeros((n, 3)), np.zeros(n, np.int32)
+        tt, tk = ctypes.c_double(), ctypes.c_double()
+        lib.gc_gpu_integrate_batch(n, 0.0, 600.0, p(C), p(cfg0["RCONST"]), p(cfg0["ATOL"]),
+                                   p(cfg0["RTOL"]), p(cfg0["ICNTRL"]), p(cfg0["RCNTRL"]),
+                                   p(perm), p(IS), p(RS), p(IE), ctypes.byref(tt),
+                                   ctypes.byref(tk))
+        _assert_bitexact((C, IS, RS, IE), ref0, mech.nvar)
 
 
 def test_hybrid_step_split_and_concurrency(cfg0, ref0, mech):

@@ -7,6 +7,7 @@
   include/gcchemgpu.h declares.
 """
 import ctypes
+from pathlib import Path
 import re
 import subprocess
 
@@ -95,4 +96,12 @@ def test_link_hygiene():
     assert exe.exists(), "build() builds the link check"
     needed = subprocess.run(["ldd", str(exe)], capture_output=True, text=True, check=True).stdout
     assert "libgcchemgpu.so" in needed and "libgomp" in needed
+    # the Fortran caller (flang -fopenmp) links it too, beside LLVM's OpenMP runtime
+    fexe = L.LIB_DIR / "gc_fortran_caller"
+    from gcchem_amd import build as B
+    if Path(B.FLANG).exists():
+        assert fexe.exists(), "build() builds the Fortran caller"
+        needed = subprocess.run(["ldd", str(fexe)], capture_output=True, text=True,
+                                check=True).stdout
+        assert "libgcchemgpu.so" in needed and "libomp" in needed
 
