@@ -231,7 +231,7 @@ def test_environment_selects_kernel_and_chunks(cfg0, ref0, mech, monkeypatch):
 
 
 def test_hybrid_step_split_and_concurrency(cfg0, ref0, mech):
-    """gc_amd_hybrid_step on the 2,048 config-0 cells in cost order (previous
+    """gc_amd_hybrid_step on 2,048 seeded fullchem cells (dt 600 s) in cost order (previous
     step counts = the oracle's Nstp): the head goes to the GPU and is
     bit-identical to the oracle, the caller's CPU solve is called once for
     exactly the tail and runs while the GPU solves, and the next step's split
