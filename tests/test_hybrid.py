@@ -174,3 +174,12 @@ def test_hybrid_step_all_on_cpu(lib):
     assert lib.gc_amd_hybrid_step(n, 0.0, 600.0, None, None, None, None, None, None,
                                   p(nsteps), None, None, None, ctypes.byref(st2),
                                   L.CPU_TAIL_FN(), None) == -1
+
+
+def test_python_cost_order_equals_native_slots(lib):
+    """The benchmark's launch order (inputs.cost_order) is the library's cost
+    slot order: the same stable descending sort."""
+    from gcchem_amd import inputs
+    prev = np.random.default_rng(9).integers(0, 120, 5000).astype(np.int32)
+    rc, slot, cell = lib_slots(lib, prev)
+    assert rc == 0 and np.array_equal(inputs.cost_order(prev), cell)
