@@ -102,7 +102,7 @@ def adjoint_ratio(mech, n=65536):
     import numpy as np
     import torch
     from gcchem_amd import adjoint, inputs, solver
-    d = inputs.config3(mech, n=n, seed=3)
+    d = inputs.adjoint_cells(mech, n=n, seed=3)
     b = solver.Batch.from_numpy(d)
     r = solver.solve(b, kernel="persistent")
     torch.cuda.synchronize()
@@ -191,7 +191,7 @@ def bench_grid_2x25(args, mech, lib, tune):
     import torch
     from gcchem_amd import inputs, solver
     t0 = time.time()
-    d = inputs.config2(mech, seed=2)
+    d = inputs.grid_2x25_cells(mech, seed=2)
     n = d["C"].shape[0]
     log(f"generated {n} cells in {time.time() - t0:.1f} s")
     b = solver.Batch.from_numpy(d)
@@ -251,7 +251,7 @@ def bench_stress(args):
     per, total_p, total_q, adj = [], 0.0, 0.0, []
     batches = []
     for i, n in enumerate(sizes):
-        d = inputs.config4(mech, n, seed=40 + i)
+        d = inputs.stress_cells(mech, n, seed=40 + i)
         b = solver.Batch.from_numpy(d)
         rp = solver.solve(b, kernel="persistent", mech="stress500")
         torch.cuda.synchronize()
@@ -347,7 +347,7 @@ def main():
         print(json.dumps(bench_stress(args)), flush=True)
         return
     t0 = time.time()
-    d = inputs.config1(mech, seed=1)
+    d = inputs.grid_4x5_cells(mech, seed=1)
     if args.cells:
         d = inputs.subset(d, np.arange(args.cells))
     n = d["C"].shape[0]

@@ -85,7 +85,7 @@ def main(argv=None):
 
     mech = mechanism.load(args.mech)
     lib = L.load(args.mech)
-    d = inputs.config1(mech, seed=1)
+    d = inputs.grid_4x5_cells(mech, seed=1)
     if args.cells:
         d = inputs.subset(d, np.arange(args.cells))
     n = d["C"].shape[0]
@@ -142,7 +142,7 @@ def main(argv=None):
     path = Path(args.out) if args.out else tuning.TUNE_DIR / f"{args.mech}.json"
     data = json.loads(path.read_text()) if path.exists() else {}
     data["launch"] = cur
-    data["autotune"] = {"workload": f"{args.mech} 4x5 grid config1 generator seed 1, {n} cells, "
+    data["autotune"] = {"workload": f"{args.mech} 4x5 grid generator (grid_4x5_cells) seed 1, {n} cells, "
                                     f"dt 600 s, cost order, persistent kernel",
                         "best_kernel_s": best_t, "cells_per_s": n / best_t,
                         "device": torch.cuda.get_device_name(0), "sweep": sweep}

@@ -1,6 +1,6 @@
 """Write the synthetic mechanism IRs under mechanisms/.
 
-* stress500.json  config-4 stress mechanism (500 species, 1500 reactions,
+* stress500.json  the stress mechanism (500 species, 1500 reactions,
                   ~3% LU fill), synthetic.stress_mechanism(seed=4)
 * auto1.json      one species, X -> 2X: with rate 1/8 and H = 16 the
                   prepared diagonal -J + 1/(0.5 H) is exactly zero, which

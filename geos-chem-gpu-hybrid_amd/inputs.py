@@ -124,7 +124,7 @@ def _assemble(mech, rng, T, P_hpa, cos_sza, n):
                 T=T, P_hpa=P_hpa, cos_sza=cos_sza)
 
 
-def config0(mech: Mechanism, n: int = 2048, seed: int = 0) -> dict:
+def random_cells(mech: Mechanism, n: int = 2048, seed: int = 0) -> dict:
     """Independent cells: T ~ U(200,310) K, P ~ logU(10,1000) hPa, SZA ~ U(0,120)
     deg, variable species logU(1e-2,1e12), dt = 600 s."""
     rng = np.random.default_rng(seed)
@@ -148,7 +148,7 @@ def std_atmosphere_T(z_km):
     return T
 
 
-def grid_config(mech: Mechanism, nlon: int, nlat: int, nlev: int, seed: int = 1) -> dict:
+def grid_cells(mech: Mechanism, nlon: int, nlat: int, nlev: int, seed: int = 1) -> dict:
     """Cells of a lon x lat x level grid (lon fastest, as GEOS-Chem's I,J,L):
     pressure log-spaced 1000 -> 0.01 hPa over the levels, standard-atmosphere
     temperature by level with +-15 K uniform noise, SZA from the cell's
@@ -174,21 +174,21 @@ def grid_config(mech: Mechanism, nlon: int, nlat: int, nlev: int, seed: int = 1)
     return d
 
 
-def config1(mech: Mechanism, seed: int = 1) -> dict:
+def grid_4x5_cells(mech: Mechanism, seed: int = 1) -> dict:
     """4x5 grid: 72 x 46 columns x 72 levels = 238,464 cells."""
-    return grid_config(mech, 72, 46, 72, seed)
+    return grid_cells(mech, 72, 46, 72, seed)
 
 
-def config2(mech: Mechanism, seed: int = 2) -> dict:
+def grid_2x25_cells(mech: Mechanism, seed: int = 2) -> dict:
     """2x2.5 grid: 144 x 91 columns x 72 levels = 943,488 cells."""
-    return grid_config(mech, 144, 91, 72, seed)
+    return grid_cells(mech, 144, 91, 72, seed)
 
 
-def config3(mech: Mechanism, n: int = 65536, seed: int = 3) -> dict:
+def adjoint_cells(mech: Mechanism, n: int = 65536, seed: int = 3) -> dict:
     """Adjoint workload: n cells taken evenly across the 4x5 grid generator's
     238,464 cells (all levels and latitudes represented), plus the loss
     weights ``w[NVAR]`` ~ N(0, 1) per species, all from `seed`."""
-    d = grid_config(mech, 72, 46, 72, seed)
+    d = grid_cells(mech, 72, 46, 72, seed)
     idx = np.linspace(0, d["C"].shape[0] - 1, n).round().astype(np.int64)
     out = subset(d, idx)
     out["w"] = np.random.default_rng(seed + 1000).standard_normal(mech.nvar)
@@ -217,7 +217,7 @@ def cost_order(nsteps, descending: bool = True):
     return (order + 1).astype(np.int32)
 
 
-def config4(mech: Mechanism, n: int, seed: int = 4, max_steps: int = 1000) -> dict:
+def stress_cells(mech: Mechanism, n: int, seed: int = 4, max_steps: int = 1000) -> dict:
     """Cells of a synthetic stress mechanism: concentrations log-uniform
     [1e-2, 1e10]; rate constants log-uniform [1e-6, 1e4] s^-1 in scaled units,
     i.e. RCONST = k / (1e6)^(order-1) so that k is the pseudo-first-order rate

@@ -17,7 +17,7 @@ from gcchem_amd import inputs
 
 @pytest.fixture(scope="module")
 def cells(mech, oracle_cr):
-    d = inputs.config0(mech, n=3, seed=3)
+    d = inputs.random_cells(mech, n=3, seed=3)
     out = []
     for c in range(3):
         C, IS, RS, IE, Hs = oracle_cr.solve_cell_rec(

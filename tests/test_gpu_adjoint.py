@@ -28,9 +28,9 @@ def _say(*a):
 
 @pytest.fixture(scope="module")
 def adj_case(mech):
-    d = inputs.config3(mech, n=1024, seed=3)
+    d = inputs.adjoint_cells(mech, n=1024, seed=3)
     b = solver.Batch.from_numpy(d)
-    _say("record solve: config3 generator, 1024 cells")
+    _say("record solve: adjoint-workload generator, 1024 cells")
     out, rec = adjoint.record_solve(b)
     torch.cuda.synchronize()
     return d, b, out, rec
@@ -54,7 +54,7 @@ def test_record_solve_matches_forward(mech, oracle_cr, adj_case):
 
 
 def test_record_pool_regrows(mech):
-    d = inputs.config0(mech, n=300, seed=8)
+    d = inputs.random_cells(mech, n=300, seed=8)
     b = solver.Batch.from_numpy(d)
     _say("record solve with a 10-record pool, 300 cells")
     out, rec = adjoint.record_solve(b, cap=10)
@@ -204,7 +204,7 @@ def test_autograd_function(mech, adj_case, adj_grads):
 
 
 def test_adjoint_non_autonomous_flagged(mech):
-    d = inputs.config0(mech, n=70, seed=2)
+    d = inputs.random_cells(mech, n=70, seed=2)
     d["ICNTRL"][:, 0] = 0
     d["ICNTRL"][:5, 0] = 1
     b = solver.Batch.from_numpy(d)

@@ -19,7 +19,7 @@ from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
-TOL_REL = 1e-9  # north-star tolerance on final concentrations above ATOL
+TOL_REL = 1e-9  # target tolerance on final concentrations above ATOL
 
 
 def _run_oracle(o, d, perm=None):
@@ -44,13 +44,13 @@ def _assert_bitexact(g, ref, nvar):
 
 
 @pytest.fixture(scope="module")
-def cfg0(mech):
-    return inputs.config0(mech, n=2048, seed=0)
+def cells2k(mech):
+    return inputs.random_cells(mech, n=2048, seed=0)
 
 
 @pytest.fixture(scope="module")
-def ref0(cfg0, oracle_cr):
-    return _run_oracle(oracle_cr, cfg0)
+def ref2k(cells2k, oracle_cr):
+    return _run_oracle(oracle_cr, cells2k)
 
 
 _LOAD_BEFORE_TORCH = """
@@ -66,7 +66,7 @@ from gcchem_amd import lib as L
 L.load("fullchem")                      # the library first, torch not imported yet
 from gcchem_amd import inputs, mechanism, solver
 import torch
-b = solver.Batch.from_numpy(inputs.config0(mechanism.load("fullchem"), n=64, seed=1))
+b = solver.Batch.from_numpy(inputs.random_cells(mechanism.load("fullchem"), n=64, seed=1))
 r = solver.solve(b, kernel="persistent")
 torch.cuda.synchronize()
 assert bool((r.IERR == 1).all())
@@ -97,7 +97,7 @@ def test_openmp_threads_around_the_library(tmp_path, mech):
     opens an OpenMP team while the library's own thread drives the device:
     the combined batch equals the single call."""
     import subprocess
-    d = inputs.config0(mech, n=256, seed=5)
+    d = inputs.random_cells(mech, n=256, seed=5)
     n = d["C"].shape[0]
     (tmp_path / "meta.txt").write_text(
         f"{n} {mech.nspec} {mech.nreact} {mech.nvar} {float(d['Tstart'])!r} {float(d['Tend'])!r}\n")
@@ -145,37 +145,37 @@ def test_selftest_and_info():
 
 
 @pytest.mark.parametrize("kernel", ["plain", "persistent"])
-def test_config0_bitexact(cfg0, ref0, mech, kernel):
-    _assert_bitexact(_gpu(cfg0, kernel), ref0, mech.nvar)
+def test_random_cells_bitexact(cells2k, ref2k, mech, kernel):
+    _assert_bitexact(_gpu(cells2k, kernel), ref2k, mech.nvar)
 
 
-def test_config0_vs_reference_pow(cfg0, mech, oracle_libm):
+def test_random_cells_vs_reference_pow(cells2k, mech, oracle_libm):
     """Against libm pow (the reference CPU build): identical IERR, identical
     step counts in >= 99.9% of cells, and final concentrations above ATOL
     within 1e-9 relative for >= 99.99% of values (max 1e-7).
 
     The GPU equals the oracle bit for bit when both round Err**(1/3)
-    correctly (test_config0_bitexact); glibc's pow misrounds about 0.08% of
+    correctly (test_random_cells_bitexact); glibc's pow misrounds about 0.08% of
     its arguments by one ulp, and a one-ulp change of a step size moves a few
     values that sit just above ATOL by up to ~1e-8 relative after ~90 steps.
     The same spread appears between the two CPU oracle modes."""
-    ref = _run_oracle(oracle_libm, cfg0)
-    C, IS, RS, IE = _gpu(cfg0, "persistent")
+    ref = _run_oracle(oracle_libm, cells2k)
+    C, IS, RS, IE = _gpu(cells2k, "persistent")
     nv = mech.nvar
     assert np.array_equal(IE, ref[3])
     same_steps = np.all(IS[:, 2:5] == ref[1][:, 2:5], axis=1).mean()
     assert same_steps >= 0.999
-    above = np.abs(ref[0][:, :nv]) > cfg0["ATOL"][None, :]
+    above = np.abs(ref[0][:, :nv]) > cells2k["ATOL"][None, :]
     rel = np.abs(C[:, :nv] - ref[0][:, :nv]) / np.abs(ref[0][:, :nv]).clip(1e-300)
     assert (rel[above] <= TOL_REL).mean() >= 0.9999
     assert rel[above].max() <= 1e-7
 
 
-def test_host_array_entry_points(cfg0, ref0, mech):
+def test_host_array_entry_points(cells2k, ref2k, mech):
     """gc_gpu_integrate_batch(_dev) with host arrays (the model's call)."""
     lib = L.load()
     n = 256
-    d = inputs.subset(cfg0, np.arange(n))
+    d = inputs.subset(cells2k, np.arange(n))
     C = np.ascontiguousarray(d["C"].copy())
     IS = np.zeros((n, 8), np.int32)
     RS = np.zeros((n, 3))
@@ -187,7 +187,7 @@ def test_host_array_entry_points(cfg0, ref0, mech):
                                    p(d["RTOL"]), p(d["ICNTRL"]), p(d["RCNTRL"]), p(perm),
                                    p(IS), p(RS), p(IE), ctypes.byref(tt), ctypes.byref(tk))
     assert tt.value > 0 and tk.value > 0
-    sub = tuple(x[:n] for x in ref0[:4])
+    sub = tuple(x[:n] for x in ref2k[:4])
     _assert_bitexact((C, IS, RS, IE), sub, mech.nvar)
     # a device that does not exist: every IERR = -99
     lib.gc_gpu_integrate_batch_dev(77, n, 0.0, 600.0, p(C), p(d["RCONST"]), p(d["ATOL"]),
@@ -201,14 +201,14 @@ def test_host_array_entry_points(cfg0, ref0, mech):
     assert tt.value == 0.0 and tk.value == 0.0
 
 
-def test_environment_selects_kernel_and_chunks(cfg0, ref0, mech, monkeypatch):
+def test_environment_selects_kernel_and_chunks(cells2k, ref2k, mech, monkeypatch):
     """The reference's run-time switches (gpulib/src/gc_gpu_batch_api.F90:
     231-256), read at every host-array call: GC_GPU_KERNEL=plain with
     GC_GPU_WORKSPACE=global solves in chunks of GC_GPU_CHUNK cells (clamped
     to [1024, nCell]); a malformed chunk size falls back to the default.
     Every setting gives the oracle's results bit for bit."""
     lib = L.load()
-    n = cfg0["C"].shape[0]
+    n = cells2k["C"].shape[0]
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     perm = np.arange(1, n + 1, dtype=np.int32)
     for env in ({"GC_GPU_KERNEL": "plain"},
@@ -220,17 +220,17 @@ def test_environment_selects_kernel_and_chunks(cfg0, ref0, mech, monkeypatch):
         for k, v in env.items():
             monkeypatch.setenv(k, v)
         print(f"case: host-array call, {n} cells, {env}", flush=True)
-        C = np.ascontiguousarray(cfg0["C"].copy())
+        C = np.ascontiguousarray(cells2k["C"].copy())
         IS, RS, IE = np.zeros((n, 8), np.int32), np.zeros((n, 3)), np.zeros(n, np.int32)
         tt, tk = ctypes.c_double(), ctypes.c_double()
-        lib.gc_gpu_integrate_batch(n, 0.0, 600.0, p(C), p(cfg0["RCONST"]), p(cfg0["ATOL"]),
-                                   p(cfg0["RTOL"]), p(cfg0["ICNTRL"]), p(cfg0["RCNTRL"]),
+        lib.gc_gpu_integrate_batch(n, 0.0, 600.0, p(C), p(cells2k["RCONST"]), p(cells2k["ATOL"]),
+                                   p(cells2k["RTOL"]), p(cells2k["ICNTRL"]), p(cells2k["RCNTRL"]),
                                    p(perm), p(IS), p(RS), p(IE), ctypes.byref(tt),
                                    ctypes.byref(tk))
-        _assert_bitexact((C, IS, RS, IE), ref0, mech.nvar)
+        _assert_bitexact((C, IS, RS, IE), ref2k, mech.nvar)
 
 
-def test_hybrid_step_split_and_concurrency(cfg0, ref0, mech):
+def test_hybrid_step_split_and_concurrency(cells2k, ref2k, mech):
     """gc_amd_hybrid_step on 2,048 seeded fullchem cells (dt 600 s) in cost order (previous
     step counts = the oracle's Nstp): the head goes to the GPU and is
     bit-identical to the oracle, the caller's CPU solve is called once for
@@ -238,13 +238,13 @@ def test_hybrid_step_split_and_concurrency(cfg0, ref0, mech):
     follows the measured rates."""
     import time
     lib = L.load()
-    n = cfg0["C"].shape[0]
-    prev = np.ascontiguousarray(ref0[1][:, 2], dtype=np.int32)
+    n = cells2k["C"].shape[0]
+    prev = np.ascontiguousarray(ref2k[1][:, 2], dtype=np.int32)
     slot, cell = np.zeros(n, np.int32), np.zeros(n, np.int32)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     assert lib.gc_amd_cost_slots(n, p(prev), 1, p(slot), p(cell)) == 0
     order = cell - 1
-    d = {k: np.ascontiguousarray(cfg0[k][order]) for k in ("C", "RCONST", "ICNTRL", "RCNTRL")}
+    d = {k: np.ascontiguousarray(cells2k[k][order]) for k in ("C", "RCONST", "ICNTRL", "RCNTRL")}
     nsteps = np.ascontiguousarray(prev[order])
     calls = []
 
@@ -260,8 +260,8 @@ def test_hybrid_step_split_and_concurrency(cfg0, ref0, mech):
         frac_rates = (st.rate_gpu / (st.rate_gpu + st.rate_cpu)
                       if st.rate_gpu > 0 and st.rate_cpu > 0 else 0.5)
         print(f"case: hybrid step {step}, {n} cells", flush=True)
-        rc = lib.gc_amd_hybrid_step(n, 0.0, 600.0, p(C), p(d["RCONST"]), p(cfg0["ATOL"]),
-                                    p(cfg0["RTOL"]), p(d["ICNTRL"]), p(d["RCNTRL"]), p(nsteps),
+        rc = lib.gc_amd_hybrid_step(n, 0.0, 600.0, p(C), p(d["RCONST"]), p(cells2k["ATOL"]),
+                                    p(cells2k["RTOL"]), p(d["ICNTRL"]), p(d["RCNTRL"]), p(nsteps),
                                     p(IS), p(RS), p(IE), ctypes.byref(st), tail, None)
         ng = st.n_gpu
         assert rc == 0 and 0 < ng < n
@@ -270,7 +270,7 @@ def test_hybrid_step_split_and_concurrency(cfg0, ref0, mech):
         assert (st.n_dev_use, st.dev_lo[0], st.dev_hi[0]) == (1, 1, ng)
         head = order[:ng]
         _assert_bitexact((C[:ng], IS[:ng], RS[:ng], IE[:ng]),
-                         tuple(x[head] for x in ref0[:4]), mech.nvar)
+                         tuple(x[head] for x in ref2k[:4]), mech.nvar)
         assert np.array_equal(C[ng:], d["C"][ng:])          # the tail is the caller's
         assert st.t_gpu > 0 and st.t_cpu >= 0.4 and st.rate_gpu > 0 and st.rate_cpu > 0
         assert st.t_region < st.t_gpu + st.t_cpu - 0.2      # the two sides overlapped
@@ -284,35 +284,35 @@ def test_hybrid_step_split_and_concurrency(cfg0, ref0, mech):
         C = d["C"].copy()
         IS, RS, IE = np.zeros((n, 8), np.int32), np.zeros((n, 3)), np.zeros(n, np.int32)
         print("case: hybrid step, 2 devices on a 1-device box", flush=True)
-        rc = lib.gc_amd_hybrid_step(n, 0.0, 600.0, p(C), p(d["RCONST"]), p(cfg0["ATOL"]),
-                                    p(cfg0["RTOL"]), p(d["ICNTRL"]), p(d["RCNTRL"]), p(nsteps),
+        rc = lib.gc_amd_hybrid_step(n, 0.0, 600.0, p(C), p(d["RCONST"]), p(cells2k["ATOL"]),
+                                    p(cells2k["RTOL"]), p(d["ICNTRL"]), p(d["RCNTRL"]), p(nsteps),
                                     p(IS), p(RS), p(IE), ctypes.byref(st), tail, None)
         lo1, hi1 = st.dev_lo[1], st.dev_hi[1]
         assert st.n_dev_use == 2 and st.dev_lo[0] == 1 and st.dev_hi[0] == lo1 - 1
         assert rc == hi1 - lo1 + 1 > 0 and np.all(IE[lo1 - 1:hi1] == -99)
         head = order[:lo1 - 1]
         _assert_bitexact((C[:lo1 - 1], IS[:lo1 - 1], RS[:lo1 - 1], IE[:lo1 - 1]),
-                         tuple(x[head] for x in ref0[:4]), mech.nvar)
+                         tuple(x[head] for x in ref2k[:4]), mech.nvar)
 
 
 @pytest.mark.parametrize("kernel", ["plain", "persistent"])
-def test_permutation_and_chunks(cfg0, ref0, mech, kernel):
+def test_permutation_and_chunks(cells2k, ref2k, mech, kernel):
     """Cells are independent: any launch order and any chunking gives the same
     per-cell results."""
-    d = dict(cfg0)
-    d["Perm"] = (np.random.default_rng(1).permutation(cfg0["C"].shape[0]) + 1).astype(np.int32)
-    _assert_bitexact(_gpu(d, kernel), ref0, mech.nvar)
-    _assert_bitexact(_gpu(cfg0, kernel, chunk=700), ref0, mech.nvar)
+    d = dict(cells2k)
+    d["Perm"] = (np.random.default_rng(1).permutation(cells2k["C"].shape[0]) + 1).astype(np.int32)
+    _assert_bitexact(_gpu(d, kernel), ref2k, mech.nvar)
+    _assert_bitexact(_gpu(cells2k, kernel, chunk=700), ref2k, mech.nvar)
 
 
 @pytest.mark.parametrize("kernel", ["plain", "persistent"])
-def test_solve_ordered_after_callers_stream(cfg0, ref0, mech, kernel):
+def test_solve_ordered_after_callers_stream(cells2k, ref2k, mech, kernel):
     """The solve reads the inputs as the caller's stream leaves them: work
     queued on torch's stream right before the call (a slow kernel, then the
     copy of the real concentrations over zeros) completes before the library
     reads C.  Regression test for a solve that started on the library's own
     stream while the copy was still queued."""
-    b = solver.Batch.from_numpy(cfg0)
+    b = solver.Batch.from_numpy(cells2k)
     good = b.C.clone()
     b.C.zero_()
     torch.cuda.synchronize()
@@ -321,10 +321,10 @@ def test_solve_ordered_after_callers_stream(cfg0, ref0, mech, kernel):
     r = solver.solve(b, kernel=kernel, inplace=True)
     torch.cuda.synchronize()
     g = (r.C.cpu().numpy(), r.ISTAT.cpu().numpy(), r.RSTAT.cpu().numpy(), r.IERR.cpu().numpy())
-    _assert_bitexact(g, ref0, mech.nvar)
+    _assert_bitexact(g, ref2k, mech.nvar)
 
 
-def test_tuning_variants_identical(cfg0, ref0, mech):
+def test_tuning_variants_identical(cells2k, ref2k, mech):
     lib = L.load()
     old = L.get_tuning(lib)
     try:
@@ -332,22 +332,22 @@ def test_tuning_variants_identical(cfg0, ref0, mech):
                   {"block": 64, "blocks_per_cu": 4},
                   {"cells_per_wave": 16, "grain": 4096, "variant": 1, "blocks_per_cu": 0}):
             L.set_tuning(lib, **t)
-            _assert_bitexact(_gpu(cfg0, "persistent"), ref0, mech.nvar)
+            _assert_bitexact(_gpu(cells2k, "persistent"), ref2k, mech.nvar)
             print(f"tuning {t} ok", flush=True)
     finally:
         L.set_tuning(lib, **old)
 
 
 @pytest.mark.parametrize("n", [1, 63, 65, 200])
-def test_ragged_sizes(cfg0, ref0, mech, n):
-    d = inputs.subset(cfg0, np.arange(n))
-    sub = tuple(x[:n] for x in ref0[:4])
+def test_ragged_sizes(cells2k, ref2k, mech, n):
+    d = inputs.subset(cells2k, np.arange(n))
+    sub = tuple(x[:n] for x in ref2k[:4])
     for kernel in ("plain", "persistent"):
         _assert_bitexact(_gpu(d, kernel), sub, mech.nvar)
 
 
 def test_failure_codes_on_gpu(mech, oracle_cr):
-    d = inputs.config0(mech, n=64, seed=21)
+    d = inputs.random_cells(mech, n=64, seed=21)
     d["ICNTRL"][0, 11] = 1   # -90
     d["ICNTRL"][1, 2] = 2    # -2
     d["ICNTRL"][2, 3] = 1    # -6
@@ -363,8 +363,8 @@ def test_failure_codes_on_gpu(mech, oracle_cr):
         _assert_bitexact(_gpu(d, kernel), ref, mech.nvar)
 
 
-def test_bad_perm_rejected(cfg0):
-    d = inputs.subset(cfg0, np.arange(64))
+def test_bad_perm_rejected(cells2k):
+    d = inputs.subset(cells2k, np.arange(64))
     d["Perm"][5] = 65  # outside 1..nCell
     b = solver.Batch.from_numpy(d)
     with pytest.raises(RuntimeError, match="Perm"):
@@ -398,19 +398,19 @@ def stress():
 
 @pytest.mark.parametrize("kernel", ["plain", "persistent"])
 def test_stress_mechanism_bitexact(stress, kernel):
-    d = inputs.config4(stress, n=256, seed=4)
+    d = inputs.stress_cells(stress, n=256, seed=4)
     d["Tend"] = 20.0
     ref = _run_oracle(Oracle(stress.ir, pow_mode="cr"), d)
     _assert_bitexact(_gpu(d, kernel, mech="stress500"), ref, stress.nvar)
 
 
 def test_stress_ragged_batches(stress):
-    """Independent batches of different sizes in turn (the config-4 ragged
+    """Independent batches of different sizes in turn (the ragged-batch
     stress, scaled down to a 20 s step): each matches the oracle on a
     sample of its cells."""
     o = Oracle(stress.ir, pow_mode="cr")
     for i, n in enumerate((1, 100, 1000)):
-        d = inputs.config4(stress, n=n, seed=40 + i)
+        d = inputs.stress_cells(stress, n=n, seed=40 + i)
         d["Tend"] = 20.0
         g = _gpu(d, "persistent", mech="stress500")
         idx = np.unique(np.linspace(0, n - 1, min(n, 16)).astype(int))
@@ -422,7 +422,7 @@ def test_stress_ragged_batches(stress):
 def test_full_4x5_grid(mech, oracle_cr):
     """238,464 cells: plain == persistent bit for bit, all IERR = 1, and 48
     sampled cells bit-identical to the oracle."""
-    d = inputs.config1(mech, seed=1)
+    d = inputs.grid_4x5_cells(mech, seed=1)
     b = solver.Batch.from_numpy(d)
     rp = solver.solve(b, kernel="persistent")
     rq = solver.solve(b, kernel="plain")

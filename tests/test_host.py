@@ -29,9 +29,9 @@ def test_cost_order_is_a_stable_descending_sort():
 
 
 def test_generators_are_seeded_and_shaped(mech):
-    a = inputs.config0(mech, n=16, seed=3)
-    b = inputs.config0(mech, n=16, seed=3)
-    c = inputs.config0(mech, n=16, seed=4)
+    a = inputs.random_cells(mech, n=16, seed=3)
+    b = inputs.random_cells(mech, n=16, seed=3)
+    c = inputs.random_cells(mech, n=16, seed=4)
     for k in ("C", "RCONST", "ICNTRL", "RCNTRL"):
         assert np.array_equal(a[k], b[k])
     assert not np.array_equal(a["C"], c["C"])
@@ -44,9 +44,9 @@ def test_generators_are_seeded_and_shaped(mech):
 
 
 def test_grid_and_adjoint_workloads(mech):
-    d = inputs.grid_config(mech, 4, 3, 5, seed=1)
+    d = inputs.grid_cells(mech, 4, 3, 5, seed=1)
     assert d["C"].shape[0] == 4 * 3 * 5
-    d3 = inputs.config3(mech, n=100, seed=3)
+    d3 = inputs.adjoint_cells(mech, n=100, seed=3)
     assert d3["C"].shape == (100, mech.nspec) and d3["w"].shape == (mech.nvar,)
     s = inputs.subset(d3, np.array([5, 2]))
     assert np.array_equal(s["C"][1], d3["C"][2]) and list(s["Perm"]) == [1, 2]
@@ -54,7 +54,7 @@ def test_grid_and_adjoint_workloads(mech):
 
 def test_stress_workload_caps_steps():
     m = mechanism.load("stress500")
-    d = inputs.config4(m, 8, seed=4, max_steps=777)
+    d = inputs.stress_cells(m, 8, seed=4, max_steps=777)
     assert np.all(d["ICNTRL"][:, 3] == 777)
     assert d["C"].shape == (8, m.nspec)
 
@@ -70,9 +70,11 @@ def test_autotune_feasibility():
     assert at.feasible({**base, "variant": 1, "blocks_per_cu": 0})       # library sizes it
 
 
-def test_operation_counts_reconcile_with_the_survey(mech):
-    """The generator's own counts of the reference expressions against the
-    survey's counts (Fun 11,139, Jac 10,607, LU 44,353, solve 11,013)."""
+def test_operation_counts_reconcile_with_the_reference_text(mech):
+    """The generator's own counts of the reference expressions against counts
+    taken independently from the text of the reference's generated Fortran
+    (Fun_SPLIT 11,139, Jac_SP 10,607, KppDecomp 44,353 = 2,773 divides +
+    2 x 20,790, KppSolve 11,013 flops)."""
     assert abs(mech.flops_fun() - 11139) / 11139 < 0.005
     assert abs(mech.flops_jac() - 10607) / 10607 < 0.005
     assert mech.flops_decomp() == 44353
@@ -85,7 +87,7 @@ def test_operation_counts_reconcile_with_the_survey(mech):
 
 
 def test_solver_rejects_host_tensors_and_bad_shapes(mech):
-    d = inputs.config0(mech, n=4, seed=0)
+    d = inputs.random_cells(mech, n=4, seed=0)
     b = solver.Batch.from_numpy(d, device="cpu")
     with pytest.raises(TypeError):
         solver._check(b, (mech.nvar, mech.nfix, mech.nreact, mech.nnz))

@@ -112,7 +112,7 @@ def test_mgpu_dummy_chemistry(mech, oracle_libm):
 
 
 def test_failure_codes(mech, oracle_libm):
-    d = inputs.config0(mech, n=8, seed=21)
+    d = inputs.random_cells(mech, n=8, seed=21)
     I = d["ICNTRL"].copy()
     Rc = d["RCNTRL"].copy()
     I[0, 11] = 1      # auto-reduce requested -> -90
@@ -190,7 +190,7 @@ def test_robertson_against_radau():
 def test_pow_modes_agree_to_tolerance(mech):
     """libm pow and the correctly rounded Err**(1/3) differ in the last ulp for
     a small fraction of arguments; the solutions stay within 1e-9."""
-    d = inputs.config0(mech, n=64, seed=7)
+    d = inputs.random_cells(mech, n=64, seed=7)
     args = (d["Tstart"], d["Tend"], d["C"], d["RCONST"], d["ATOL"], d["RTOL"], d["ICNTRL"], d["RCNTRL"])
     a = Oracle(mech.ir, pow_mode="libm").integrate_batch(*args)
     b = Oracle(mech.ir, pow_mode="cr").integrate_batch(*args)

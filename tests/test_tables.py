@@ -538,7 +538,7 @@ def test_adjoint_step_tables_match_reference(name, checkforms):
     orc = Oracle(mech.ir, pow_mode="cr")
     ir = mech.ir
     n, nf, nr = ir["nvar"], ir["nfix"], ir["nreact"]
-    d = inputs.config0(mech, n=3, seed=5) if name == "fullchem" else inputs.config4(mech, 3, seed=5)
+    d = inputs.random_cells(mech, n=3, seed=5) if name == "fullchem" else inputs.stress_cells(mech, 3, seed=5)
     if name != "fullchem":
         d["Tend"] = 20.0
     lanes = 3
