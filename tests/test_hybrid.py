@@ -183,3 +183,36 @@ def test_python_cost_order_equals_native_slots(lib):
     prev = np.random.default_rng(9).integers(0, 120, 5000).astype(np.int32)
     rc, slot, cell = lib_slots(lib, prev)
     assert rc == 0 and np.array_equal(inputs.cost_order(prev), cell)
+
+
+def test_split_converges_to_the_rate_ratio(lib):
+    """Plan -> timed solve -> update, repeated: with a GPU that processes
+    work units 3x as fast as the CPU, the fraction of work given to the GPU
+    settles at 3/4 and both sides then finish together (the reference's runs
+    converged to 0.73-0.79 on its hardware, docs/HYBRID_RESULTS.md:156-162).
+    Timings are the exact work/rate of each side."""
+    rng = np.random.default_rng(11)
+    n = 20000
+    nsteps = np.sort(rng.integers(1, 300, n))[::-1].astype(np.int32)   # cost order
+    r_gpu, r_cpu = 3.0e6, 1.0e6                                         # work units / s
+    frac, rg, rc = ctypes.c_double(0.5), ctypes.c_double(-1.0), ctypes.c_double(-1.0)
+    dr = np.full(1, -1.0)
+    work = np.zeros(n + 1, np.int64)
+    lo, hi = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    ng, nd = ctypes.c_int(), ctypes.c_int()
+    fracs = []
+    for _ in range(6):
+        assert lib.gc_amd_hybrid_plan(n, p(nsteps), 0, ctypes.byref(frac), rg.value, rc.value,
+                                      1, p(dr), p(work), ctypes.byref(ng), ctypes.byref(nd),
+                                      p(lo), p(hi)) == 0
+        t_gpu = work[ng.value] / r_gpu
+        t_cpu = (work[n] - work[ng.value]) / r_cpu
+        lib.gc_amd_hybrid_update(n, ng.value, p(work), nd.value, p(lo), p(hi),
+                                 p(np.array([t_gpu])), t_gpu, t_cpu, ctypes.byref(rg),
+                                 ctypes.byref(rc), p(dr))
+        fracs.append(frac.value)
+    assert fracs[0] == 0.5 and fracs[1] == pytest.approx(0.75, rel=1e-12)
+    assert rg.value == pytest.approx(r_gpu) and rc.value == pytest.approx(r_cpu)
+    assert fracs[-1] == pytest.approx(0.75, rel=1e-12)
+    t_gpu, t_cpu = work[ng.value] / r_gpu, (work[n] - work[ng.value]) / r_cpu
+    assert abs(t_gpu - t_cpu) <= 300 / r_cpu + 300 / r_gpu     # within one cell's work
