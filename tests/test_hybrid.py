@@ -216,3 +216,32 @@ def test_split_converges_to_the_rate_ratio(lib):
     assert fracs[-1] == pytest.approx(0.75, rel=1e-12)
     t_gpu, t_cpu = work[ng.value] / r_gpu, (work[n] - work[ng.value]) / r_cpu
     assert abs(t_gpu - t_cpu) <= 300 / r_cpu + 300 / r_gpu     # within one cell's work
+
+
+@pytest.mark.parametrize("sanitize", ["address,undefined", "thread"])
+def test_multi_device_step_under_sanitizers(tmp_path, sanitize):
+    """gc_amd_hybrid_step's multi-device path, which a one-device box cannot
+    run: csrc/gc_hybrid.cpp is linked with a stand-in device solve
+    (tests/hybrid_threads_check.cpp) and built with g++ and the host
+    sanitizers.  Every head slot is solved once by the device whose slice
+    holds it, at the right offsets of every array; the devices run at the
+    same time; a device that cannot be selected returns its slice as IERR -99
+    and is counted; empty slices are skipped; the tail goes to the callback."""
+    import shutil
+    import subprocess
+    from conftest import REPO
+    if not shutil.which("g++"):
+        pytest.skip("no g++")
+    exe = tmp_path / "hybrid_threads_check"
+    flags = [f"-fsanitize={sanitize}", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"]
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", *flags, "-pthread",
+                    f"-I{REPO / 'include'}", str(REPO / "tests" / "hybrid_threads_check.cpp"),
+                    str(REPO / "geos-chem-gpu-hybrid_amd" / "csrc" / "gc_hybrid.cpp"),
+                    "-o", str(exe)], check=True)
+    cases = [(3000, 4, -1), (3000, 4, 2), (3000, 1, -1), (5, 8, -1), (1, 3, -1), (0, 2, -1),
+             (7, 16, 3)] if sanitize != "thread" else [(3000, 4, -1), (3000, 16, 5)]
+    for n, ndev, bad in cases:
+        r = subprocess.run([str(exe), str(n), str(ndev), str(bad)], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, (n, ndev, bad, r.stdout, r.stderr[-3000:])
+        assert r.stdout.strip().splitlines()[-1] == "OK"
