@@ -5,6 +5,9 @@
 * auto1.json      one species, X -> 2X: with rate 1/8 and H = 16 the
                   prepared diagonal -J + 1/(0.5 H) is exactly zero, which
                   drives KppDecomp's singular-matrix retry on purpose
+* quad1.json      one species, 2X -> (nothing) and F -> X: a scalar
+                  nonlinear problem whose step control rejects attempts
+                  before and after the first accepted step
 """
 import importlib.util
 import json
@@ -28,6 +31,8 @@ def main():
     irs = {
         "stress500": s.stress_mechanism(),
         "auto1": s.build_ir("auto1", 1, 1, [s.Reaction({0: 1}, {0: 2.0})], reorder=False),
+        "quad1": s.build_ir("quad1", 1, 1, [s.Reaction({0: 2}, {}),
+                                            s.Reaction({}, {0: 1.0}, {0: 1})], reorder=False),
     }
     for name, ir in irs.items():
         (out / f"{name}.json").write_text(json.dumps(ir, separators=(",", ":")))

@@ -106,12 +106,42 @@ struct DevBuf {
   }
 };
 
+// The chunk loop of one launch shape, captured as a graph and instantiated
+// once; later calls of the same shape replay it (updated in place when only
+// the arguments differ).
+struct ChunkGraph {
+  hipGraphExec_t exec = nullptr;
+  int kernel = -1, span = 0, nCell = 0;
+  unsigned S = 0;
+  Tuning t;
+  GcBatchArgs args{};
+  void* ws = nullptr;
+  void* queue = nullptr;
+  bool same_shape(int k, int sp, int n, unsigned s, const Tuning& u) const {
+    return exec && kernel == k && span == sp && nCell == n && S == s && t.block == u.block &&
+           t.blocks_per_cu == u.blocks_per_cu && t.cells_per_wave == u.cells_per_wave &&
+           t.grain == u.grain && t.variant == u.variant;
+  }
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
+};
+
+bool same_args(const GcBatchArgs& a, const GcBatchArgs& b) {
+  return a.nCell == b.nCell && a.Tstart == b.Tstart && a.Tend == b.Tend && a.C == b.C &&
+         a.RCONST == b.RCONST && a.ATOL == b.ATOL && a.RTOL == b.RTOL && a.ICNTRL == b.ICNTRL &&
+         a.RCNTRL == b.RCNTRL && a.Perm == b.Perm && a.ISTAT == b.ISTAT && a.RSTAT == b.RSTAT &&
+         a.IERR == b.IERR;
+}
+
 struct DevCtx {
   std::mutex mu;
   bool init = false;
   hipStream_t stream = nullptr;
   int n_cu = 0;
   DevBuf ws, queue, flag;
+  ChunkGraph graph;
   // staging for the host-array entry points
   DevBuf C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -253,36 +283,65 @@ int device_solve(DevCtx& c, int kernel, int chunk, const GcBatchArgs& b, hipStre
   }
   if ((e = c.queue.ensure(256)) != hipSuccess) { set_error("hipMalloc(queue)", e); return -2; }
 
-  if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
   if (span == b.nCell) {
+    if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
     e = launch_chunk(c, kernel, t, b, S, st);
     if (e != hipSuccess) { set_error("launch", e); return -3; }
   } else {
-    // chunk loop over the launch order, captured once and replayed as a graph
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
-      set_error("hipStreamBeginCapture", e);
-      return -3;
+    // The chunk loop over the launch order as one graph: captured and
+    // instantiated for a new launch shape, updated in place for new
+    // arguments, replayed as it is when nothing changed.
+    ChunkGraph& cg = c.graph;
+    const bool shape = cg.same_shape(kernel, span, b.nCell, S, t);
+    if (!(shape && same_args(cg.args, b) && cg.ws == c.ws.p && cg.queue == c.queue.p)) {
+      hipGraph_t g = nullptr;
+      if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+        set_error("hipStreamBeginCapture", e);
+        return -3;
+      }
+      hipError_t le = hipSuccess;
+      for (int c0 = 0; c0 < b.nCell && le == hipSuccess; c0 += span) {
+        GcBatchArgs sub = b;
+        sub.Perm = b.Perm + c0;
+        sub.nCell = (b.nCell - c0 < span) ? (b.nCell - c0) : span;
+        le = launch_chunk(c, kernel, t, sub, S, st);
+      }
+      e = hipStreamEndCapture(st, &g);
+      if (le != hipSuccess || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        set_error("chunk capture", le != hipSuccess ? le : e);
+        return -3;
+      }
+      bool updated = false;
+      if (shape) {
+        hipGraphNode_t bad_node = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        updated = hipGraphExecUpdate(cg.exec, g, &bad_node, &res) == hipSuccess &&
+                  res == hipGraphExecUpdateSuccess;
+        (void)hipGetLastError();  // a refused update leaves no error behind
+      }
+      if (!updated) {
+        cg.release();
+        e = hipGraphInstantiate(&cg.exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+          cg.exec = nullptr;
+          (void)hipGraphDestroy(g);
+          set_error("chunk graph instantiate", e);
+          return -3;
+        }
+      }
+      (void)hipGraphDestroy(g);
+      cg.kernel = kernel;
+      cg.span = span;
+      cg.nCell = b.nCell;
+      cg.S = S;
+      cg.t = t;
+      cg.args = b;
+      cg.ws = c.ws.p;
+      cg.queue = c.queue.p;
     }
-    hipError_t le = hipSuccess;
-    for (int c0 = 0; c0 < b.nCell && le == hipSuccess; c0 += span) {
-      GcBatchArgs sub = b;
-      sub.Perm = b.Perm + c0;
-      sub.nCell = (b.nCell - c0 < span) ? (b.nCell - c0) : span;
-      le = launch_chunk(c, kernel, t, sub, S, st);
-    }
-    e = hipStreamEndCapture(st, &g);
-    if (le != hipSuccess || e != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      set_error("chunk capture", le != hipSuccess ? le : e);
-      return -3;
-    }
-    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    if (e == hipSuccess) e = hipGraphLaunch(ge, st);
-    if (ge) (void)hipGraphExecDestroy(ge);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) { set_error("chunk graph", e); return -3; }
+    if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+    if ((e = hipGraphLaunch(cg.exec, st)) != hipSuccess) { set_error("chunk graph", e); return -3; }
   }
   if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
   if ((e = hipStreamSynchronize(st)) != hipSuccess) { set_error("solve", e); return -3; }
@@ -681,6 +740,7 @@ void gc_amd_release(void) {
   DevCtx* cp = current_ctx(&d);
   if (!cp) return;
   std::lock_guard<std::mutex> lk(cp->mu);
+  cp->graph.release();
   DevBuf* bufs[] = {&cp->ws, &cp->queue, &cp->flag, &cp->C, &cp->RCONST, &cp->ATOL, &cp->RTOL,
                     &cp->ICNTRL, &cp->RCNTRL, &cp->Perm, &cp->ISTAT, &cp->RSTAT, &cp->IERR};
   for (DevBuf* b : bufs) b->release();

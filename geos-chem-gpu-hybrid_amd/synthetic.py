@@ -189,6 +189,29 @@ def _min_degree_order(adj, n):
 
 def stress_mechanism(nvar: int = 500, nreact: int = 1500, seed: int = 4,
                      nhub: int = 8) -> dict:
+    """The stress mechanism's IR (reactions of stress_reactions)."""
+    reactions, kinds_out = stress_reactions(nvar, nreact, seed, nhub)
+    ir = build_ir(f"stress{nvar}", nvar, 1, reactions)
+    ir["reaction_kind"] = kinds_out
+    return ir
+
+
+def stoichiometry(reactions: list[Reaction], nvar: int, order=None) -> np.ndarray:
+    """Net stoichiometric matrix S[i, r] = products - reactants of a reaction
+    list, with species renumbered by ``order`` (an IR's species_order: new
+    index k holds old species order[k])."""
+    new = {old: k for k, old in enumerate(order)} if order is not None else None
+    S = np.zeros((nvar, len(reactions)))
+    for r, rx in enumerate(reactions):
+        for s, m in rx.reactants.items():
+            S[new[s] if new else s, r] -= m
+        for s, c in rx.products.items():
+            S[new[s] if new else s, r] += c
+    return S
+
+
+def stress_reactions(nvar: int = 500, nreact: int = 1500, seed: int = 4,
+                     nhub: int = 8):
     """Seeded stiff stress mechanism in the shape of tropospheric chemistry:
     ``nhub`` radical hubs (the OH/HO2/NO/NO2/O3 role) react with the other
     ("leaf") species, which form oxidation chains.  Kinds: 40% bimolecular
@@ -230,6 +253,4 @@ def stress_mechanism(nvar: int = 500, nreact: int = 1500, seed: int = 4,
             pr[h] = pr.get(h, 0.0) + 1.0
         reactions.append(Reaction(re, pr))
         kinds_out.append(kind)
-    ir = build_ir(f"stress{nvar}", nvar, 1, reactions)
-    ir["reaction_kind"] = kinds_out
-    return ir
+    return reactions, kinds_out

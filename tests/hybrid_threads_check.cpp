@@ -3,12 +3,15 @@
 // g++ and the address/undefined or thread sanitizer.
 //
 // The device solve is replaced by a stand-in with the same signature: it
-// records which slice each device received, holds the call open for a time
-// proportional to the slice so that devices overlap, and writes per-cell
-// outputs that name the device and the slot, so the check can prove that
-// every slot of the head was solved exactly once by the device whose slice
-// holds it, at the right offsets of every array.  Device `bad` fails the way
-// the library does for a device that cannot be selected (IERR -99).
+// records which slice each device received, waits inside the call until
+// every device call of the step has entered (a rendezvous with a bounded
+// wait: the check plans the split first, so it knows how many device calls
+// to expect), and writes per-cell outputs that name the device and the slot,
+// so the check can prove that the device calls were in progress at the same
+// time and that every slot of the head was solved exactly once by the device
+// whose slice holds it, at the right offsets of every array.  Device `bad`
+// fails the way the library does for a device that cannot be selected
+// (IERR -99).
 //
 // usage: hybrid_threads_check <nCell> <ndev> <bad device or -1>
 // Prints one line per check and exits 0 when all pass.
@@ -28,6 +31,9 @@ const int kNvar = 5, kNfix = 2, kNreact = 7, kNspec = kNvar + kNfix;
 int g_bad = -1;
 std::atomic<int> g_active{0}, g_max_active{0};
 std::mutex g_mu;
+int g_expected = 0;                   // device calls the step's plan makes
+std::atomic<int> g_arrived{0};        // device calls that have entered
+std::atomic<bool> g_met{true};        // every call saw all of them inside
 struct Call {
   int dev, n;
   long c_off, r_off;
@@ -57,7 +63,16 @@ extern "C" void gc_gpu_integrate_batch_dev(int dev, int nCell, double, double, d
     std::lock_guard<std::mutex> lk(g_mu);
     g_calls.push_back({dev, nCell, (long)(C - g_C0), (long)(RCONST - g_R0)});
   }
-  std::this_thread::sleep_for(std::chrono::microseconds(200 * nCell));
+  // stay inside until every device call of the step has entered, at most 20 s
+  ++g_arrived;
+  const auto deadline = std::chrono::steady_clock::now() + std::chrono::seconds(20);
+  while (g_arrived.load() < g_expected) {
+    if (std::chrono::steady_clock::now() > deadline) {
+      g_met = false;
+      break;
+    }
+    std::this_thread::yield();
+  }
   for (int i = 0; i < nCell; ++i) {
     const int c = Perm[i] - 1;  // identity on the slice
     if (dev == g_bad) {
@@ -96,6 +111,16 @@ int main(int argc, char** argv) {
   st.ndev = ndev;
   st.rate_gpu = st.rate_cpu = -1.0;
   for (int d = 0; d < GC_AMD_MAX_DEV; ++d) st.dev_rate[d] = -1.0;
+  {
+    // the step's own plan (same inputs): how many device calls to expect
+    double frac = st.work_frac;
+    std::vector<long long> work((size_t)n + 1);
+    int ng = 0, nd = 0, lo[GC_AMD_MAX_DEV], hi[GC_AMD_MAX_DEV];
+    if (ndev >= 1 && ndev <= GC_AMD_MAX_DEV &&
+        gc_amd_hybrid_plan(n, nsteps.data(), st.fixed_frac, &frac, st.rate_gpu, st.rate_cpu,
+                           ndev, st.dev_rate, work.data(), &ng, &nd, lo, hi) == 0)
+      for (int d = 0; d < nd; ++d) g_expected += hi[d] >= lo[d];
+  }
   struct Tail {
     int calls = 0, first = 0, last = 0;
   } tail;
@@ -142,12 +167,13 @@ int main(int argc, char** argv) {
   for (int s = ng + 1; s <= n; ++s) fails += IERR[s - 1] != 0 || ISTAT[(size_t)(s - 1) * 8 + 2] != -1;
   fails += rc != expect_fail;
   fails += tail.calls != (ng < n) || (ng < n && (tail.first != ng + 1 || tail.last != n));
-  if (nonempty > 1) fails += g_max_active.load() < 2;  // the devices overlapped
+  // the devices overlapped: every call waited for all of them
+  fails += nonempty != g_expected || !g_met.load() || g_max_active.load() != nonempty;
   fails += (ng > 0 && !(st.rate_gpu > 0.0)) || (ng < n && !(st.rate_cpu > 0.0));
   printf("slices:");
   for (int d = 0; d < st.n_dev_use; ++d) printf(" [%d,%d]", st.dev_lo[d], st.dev_hi[d]);
-  printf("\ntail: %d call(s), slots %d-%d; failed device slots %d\n", tail.calls, tail.first,
-         tail.last, expect_fail);
+  printf("\ntail: %d call(s), slots %d-%d; failed device slots %d; rendezvous of %d %s\n",
+         tail.calls, tail.first, tail.last, expect_fail, g_expected, g_met.load() ? "met" : "timed out");
   printf(fails ? "FAIL\n" : "OK\n");
   return fails ? 1 : 0;
 }

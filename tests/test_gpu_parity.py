@@ -35,6 +35,13 @@ def _gpu(d, kernel, mech="fullchem", chunk=0):
             r.IERR.cpu().numpy())
 
 
+def _gpu_batch(b, kernel, mech="fullchem", chunk=0):
+    r = solver.solve(b, kernel=kernel, chunk=chunk, mech=mech)
+    torch.cuda.synchronize()
+    return (r.C.cpu().numpy(), r.ISTAT.cpu().numpy(), r.RSTAT.cpu().numpy(),
+            r.IERR.cpu().numpy())
+
+
 def _assert_bitexact(g, ref, nvar):
     C, IS, RS, IE = g
     assert np.array_equal(IE, ref[3])
@@ -150,25 +157,30 @@ def test_random_cells_bitexact(cells2k, ref2k, mech, kernel):
 
 
 def test_random_cells_vs_reference_pow(cells2k, mech, oracle_libm):
-    """Against libm pow (the reference CPU build): identical IERR, identical
-    step counts in >= 99.9% of cells, and final concentrations above ATOL
-    within 1e-9 relative for >= 99.99% of values (max 1e-7).
+    """Against libm pow (the reference CPU build), per cell as the target
+    states it: a cell passes when its accepted and rejected step counts are
+    identical and every final concentration above ATOL is within 1e-9
+    relative; at least 99.9% of the cells pass, and IERR is identical in
+    every cell.
 
     The GPU equals the oracle bit for bit when both round Err**(1/3)
-    correctly (test_random_cells_bitexact); glibc's pow misrounds about 0.08% of
-    its arguments by one ulp, and a one-ulp change of a step size moves a few
-    values that sit just above ATOL by up to ~1e-8 relative after ~90 steps.
-    The same spread appears between the two CPU oracle modes."""
+    correctly (test_random_cells_bitexact); glibc's pow misrounds about 0.08%
+    of its arguments by one ulp, and a one-ulp change of a step size moves a
+    few values that sit just above ATOL by up to ~1e-8 relative after ~90
+    steps (one cell of the 2,048), the same spread as between the two CPU
+    oracle modes."""
     ref = _run_oracle(oracle_libm, cells2k)
     C, IS, RS, IE = _gpu(cells2k, "persistent")
     nv = mech.nvar
     assert np.array_equal(IE, ref[3])
-    same_steps = np.all(IS[:, 2:5] == ref[1][:, 2:5], axis=1).mean()
-    assert same_steps >= 0.999
+    same_steps = np.all(IS[:, 3:5] == ref[1][:, 3:5], axis=1)
     above = np.abs(ref[0][:, :nv]) > cells2k["ATOL"][None, :]
     rel = np.abs(C[:, :nv] - ref[0][:, :nv]) / np.abs(ref[0][:, :nv]).clip(1e-300)
-    assert (rel[above] <= TOL_REL).mean() >= 0.9999
-    assert rel[above].max() <= 1e-7
+    conc_ok = np.all(np.where(above, rel <= TOL_REL, True), axis=1)
+    passed = same_steps & conc_ok
+    print(f"cells passing: {passed.mean():.5f} ({int((~passed).sum())} of {passed.size} off), "
+          f"largest difference above ATOL {rel[above].max():.3g}", flush=True)
+    assert passed.mean() >= 0.999
 
 
 def test_host_array_entry_points(cells2k, ref2k, mech):
@@ -305,6 +317,27 @@ def test_permutation_and_chunks(cells2k, ref2k, mech, kernel):
     _assert_bitexact(_gpu(cells2k, kernel, chunk=700), ref2k, mech.nvar)
 
 
+def test_chunk_graph_replays(cells2k, ref2k, mech):
+    """The chunk loop's graph is instantiated once per launch shape and
+    replayed: three chunked solves of the same batch (the cached graph as it
+    is), with eager solves and torch work between them, then a second batch
+    at other addresses (the graph updated in place), then another chunk size
+    (a new graph).  Every replay equals the oracle bit for bit."""
+    b = solver.Batch.from_numpy(cells2k)
+    for rep in range(3):
+        print(f"case: chunk graph replay {rep}, 2048 cells, chunk 600", flush=True)
+        _assert_bitexact(_gpu_batch(b, "persistent", chunk=600), ref2k, mech.nvar)
+        _assert_bitexact(_gpu_batch(b, "plain"), ref2k, mech.nvar)      # eager work between
+        torch.cuda._sleep(1_000_000)
+    b2 = solver.Batch.from_numpy(cells2k)
+    print("case: chunk graph updated for new addresses", flush=True)
+    _assert_bitexact(_gpu_batch(b2, "persistent", chunk=600), ref2k, mech.nvar)
+    _assert_bitexact(_gpu_batch(b, "persistent", chunk=600), ref2k, mech.nvar)
+    print("case: chunk graph for another chunk size", flush=True)
+    _assert_bitexact(_gpu_batch(b, "persistent", chunk=1000), ref2k, mech.nvar)
+    _assert_bitexact(_gpu_batch(b, "plain", chunk=1000), ref2k, mech.nvar)
+
+
 @pytest.mark.parametrize("kernel", ["plain", "persistent"])
 def test_solve_ordered_after_callers_stream(cells2k, ref2k, mech, kernel):
     """The solve reads the inputs as the caller's stream leaves them: work
@@ -371,24 +404,82 @@ def test_bad_perm_rejected(cells2k):
         solver.solve(b, kernel="persistent")
 
 
-def test_singular_retry_path():
-    """X -> 2X at rate k = 1/8 (J = +1/8) with Hstart = 16: the prepared
-    diagonal -J + 1/(0.5 H) is exactly 0, KppDecomp reports it singular, H is
-    halved and the attempt succeeds (Nsng = 1)."""
-    from gcchem_amd import synthetic
-    import json
-    ir = synthetic.build_ir("auto1", 1, 1, [synthetic.Reaction({0: 1}, {0: 2.0})], reorder=False)
+@pytest.mark.parametrize("kernel", ["plain", "persistent"])
+def test_singular_retry_path(kernel):
+    """X -> 2X at rate k = 1/8 (J = +1/8) with Hstart = 16 (the auto1
+    mechanism's library): the prepared diagonal -J + 1/(0.5 H) is exactly 0,
+    KppDecomp reports it singular, H is halved and the attempt succeeds
+    (gckpp_BatchIntegrator.F90:572-592).  A batch of such cells with other
+    rates and warm starts beside it: bit-exact with the oracle, the
+    singular-retry counter ISTAT(8) >= 1 where the diagonal vanishes."""
+    ir = mechanism.load("auto1").ir
     o = Oracle(ir, pow_mode="cr")
-    C = np.array([[1.0, 0.0]])
-    R = np.array([[0.125]])
-    I = np.zeros((1, 20), np.int32)
-    I[0, 0], I[0, 2] = 1, 4
-    Rc = np.zeros((1, 20))
-    Rc[0, 2] = 16.0
-    ref = o.integrate_batch(0.0, 64.0, C, R, np.array([1.0]), np.array([1e-3]), I, Rc)
-    assert ref[3][0] == 1 and ref[1][0, 7] >= 1
-    # the stress mechanism library checks the generic path; this 1-species
-    # mechanism is exercised on the CPU only (no library is built for it)
+    n = 130
+    rng = np.random.default_rng(5)
+    C = np.stack([np.exp(rng.uniform(0.0, 5.0, n)), np.ones(n)], axis=1)
+    C[0, 0] = 1.0
+    R = rng.uniform(0.01, 0.2, (n, 1))
+    R[:n // 2, 0] = 0.125
+    I = np.zeros((n, 20), np.int32)
+    I[:, 0], I[:, 2] = 1, 4
+    Rc = np.zeros((n, 20))
+    Rc[:n // 2, 2] = 16.0           # -J + 2/H == 0 at H = 16 when k = 1/8
+    Rc[n // 2:, 2] = rng.uniform(0.5, 20.0, n - n // 2)
+    d = dict(C=C, RCONST=R, ICNTRL=I, RCNTRL=Rc, ATOL=np.array([1.0]), RTOL=np.array([1e-3]),
+             Perm=np.arange(1, n + 1, dtype=np.int32), Tstart=0.0, Tend=64.0)
+    ref = _run_oracle(o, d)
+    assert np.all(ref[3] == 1) and np.all(ref[1][:n // 2, 7] >= 1)
+    print(f"case: auto1 singular retry, {n} cells, {kernel}", flush=True)
+    g = _gpu(d, kernel, mech="auto1")
+    _assert_bitexact(g, ref, 1)
+    assert np.all(g[1][:n // 2, 7] >= 1)
+
+
+def test_step_too_small_on_gpu(mech, oracle_cr):
+    """T + 0.1*H == T at T = 1e20: IERR -7 (gckpp_BatchIntegrator.F90:541-545)
+    before any step with the default Hstart, and after an accepted step and
+    rejections for a cell warm-started with H = the 1e7 s span, from both
+    kernels."""
+    d = inputs.random_cells(mech, n=70, seed=13)
+    d["Tstart"], d["Tend"] = 1e20, 1e20 + 1e7
+    d["RCNTRL"][:, 2] = 0.0
+    d["RCNTRL"][3, 2] = 1e7          # one cell with a step that T can hold
+    ref = _run_oracle(oracle_cr, d)
+    assert np.all(ref[3] == -7) and ref[1][3, 3] >= 1 and np.all(ref[1][:3] == 0)
+    for kernel in ("plain", "persistent"):
+        print(f"case: step too small, 70 cells, {kernel}", flush=True)
+        _assert_bitexact(_gpu(d, kernel), ref, mech.nvar)
+
+
+def test_step_control_rejections_on_gpu():
+    """The scalar step-control case of the CPU known-answer test (quad1
+    mechanism: 2X -> nothing, F -> X) and 255 variants of it with other rates,
+    start values and warm starts: rejections before and after the first
+    accepted step (Nrej counts only the latter), H*FacRej on repeated
+    rejections, Hnew clipped to H after a rejection.  Both kernels are
+    bit-exact with the oracle, and the known case gives ISTAT =
+    (42, 10, 16, 10, 3, 16, 64, 0)."""
+    ir = mechanism.load("quad1").ir
+    o = Oracle(ir, pow_mode="cr")
+    n = 256
+    rng = np.random.default_rng(17)
+    C = np.stack([np.exp(rng.uniform(0.0, np.log(1e4), n)), np.ones(n)], axis=1)
+    R = np.stack([np.exp(rng.uniform(np.log(1e-4), np.log(1.0), n)),
+                  np.exp(rng.uniform(np.log(1e-3), np.log(10.0), n))], axis=1)
+    I = np.zeros((n, 20), np.int32)
+    I[:, 0], I[:, 2] = 1, 4
+    Rc = np.zeros((n, 20))
+    Rc[:, 2] = np.exp(rng.uniform(0.0, np.log(300.0), n))
+    C[0, 0], R[0] = 100.0, (1e-3, 0.01)
+    Rc[0, 2] = 300.0
+    d = dict(C=C, RCONST=R, ICNTRL=I, RCNTRL=Rc, ATOL=np.array([1e-8]), RTOL=np.array([1e-4]),
+             Perm=np.arange(1, n + 1, dtype=np.int32), Tstart=0.0, Tend=100.0)
+    ref = _run_oracle(o, d)
+    assert list(ref[1][0]) == [42, 10, 16, 10, 3, 16, 64, 0]
+    assert np.sum(ref[1][:, 4] > 0) > 20 and np.sum(ref[1][:, 2] - ref[1][:, 3] > ref[1][:, 4]) > 20
+    for kernel in ("plain", "persistent"):
+        print(f"case: quad1 step control, {n} cells, {kernel}", flush=True)
+        _assert_bitexact(_gpu(d, kernel, mech="quad1"), ref, 1)
 
 
 @pytest.fixture(scope="module")
@@ -433,5 +524,34 @@ def test_full_4x5_grid(mech, oracle_cr):
     assert np.all(IE == 1)
     idx = np.random.default_rng(2).choice(IE.size, 48, replace=False)
     ref = _run_oracle(oracle_cr, inputs.subset(d, idx))
+    g = (rp.C.cpu().numpy()[idx], rp.ISTAT.cpu().numpy()[idx], rp.RSTAT.cpu().numpy()[idx], IE[idx])
+    _assert_bitexact(g, ref, mech.nvar)
+
+
+@pytest.mark.timeout(900)
+def test_full_2x25_grid(mech, oracle_cr):
+    """943,488 cells (the 2x2.5 grid): the persistent kernel in 262,144-cell
+    chunks (the chunk loop replayed as one HIP graph) equals the plain kernel
+    over the whole grid in one launch bit for bit, every IERR = 1, and 48
+    sampled cells are bit-identical to the oracle."""
+    print("case: 2x2.5 grid, generating 943,488 cells", flush=True)
+    d = inputs.grid_2x25_cells(mech, seed=2)
+    n = d["C"].shape[0]
+    idx = np.sort(np.random.default_rng(3).choice(n, 48, replace=False))
+    sub = inputs.subset(d, idx)
+    b = solver.Batch.from_numpy(d)
+    del d
+    print("case: 2x2.5 grid, persistent in chunks of 262,144", flush=True)
+    rp = solver.solve(b, kernel="persistent", chunk=262144)
+    torch.cuda.synchronize()
+    print(f"case: 2x2.5 grid, plain in one launch (persistent took {rp.t_kernel:.1f} s)",
+          flush=True)
+    rq = solver.solve(b, kernel="plain")
+    torch.cuda.synchronize()
+    for a, c in ((rp.C, rq.C), (rp.ISTAT, rq.ISTAT), (rp.RSTAT, rq.RSTAT), (rp.IERR, rq.IERR)):
+        assert torch.equal(a, c)
+    IE = rp.IERR.cpu().numpy()
+    assert np.all(IE == 1)
+    ref = _run_oracle(oracle_cr, sub)
     g = (rp.C.cpu().numpy()[idx], rp.ISTAT.cpu().numpy()[idx], rp.RSTAT.cpu().numpy()[idx], IE[idx])
     _assert_bitexact(g, ref, mech.nvar)

@@ -239,7 +239,7 @@ def test_multi_device_step_under_sanitizers(tmp_path, sanitize):
                     str(REPO / "geos-chem-gpu-hybrid_amd" / "csrc" / "gc_hybrid.cpp"),
                     "-o", str(exe)], check=True)
     cases = [(3000, 4, -1), (3000, 4, 2), (3000, 1, -1), (5, 8, -1), (1, 3, -1), (0, 2, -1),
-             (7, 16, 3)] if sanitize != "thread" else [(3000, 4, -1), (3000, 16, 5)]
+             (7, 16, 1)] if sanitize != "thread" else [(3000, 4, -1), (3000, 16, 5)]
     for n, ndev, bad in cases:
         r = subprocess.run([str(exe), str(n), str(ndev), str(bad)], capture_output=True,
                            text=True, timeout=120)

@@ -200,3 +200,125 @@ def test_pow_modes_agree_to_tolerance(mech):
     big = np.abs(a[0][:, :nv]) > 1e-2
     rel = np.abs(a[0][:, :nv] - b[0][:, :nv]) / np.maximum(np.abs(a[0][:, :nv]), 1e-300)
     assert rel[big].max() <= 1e-9
+
+
+# ---- step control: rejections before and after the first acceptance -------
+QUAD1 = dict(k=1e-3, p=0.01, y0=100.0, Tend=100.0, Hstart=300.0, atol=1e-8, rtol=1e-4)
+
+
+def rodas3_scalar_by_hand(f, J, y0, Tend, Hstart, atol, rtol):
+    """Rodas3 for one species (NVAR = 1) written out from the tableau and the
+    control flow of gckpp_BatchIntegrator.F90:464-707, with RCNTRL = 0 except
+    Hstart: the 1x1 "LU" is the prepared diagonal -J + 1/(0.5 H), each stage
+    is one division.  Returns (y, ISTAT, RSTAT, attempts) with attempts =
+    [(accepted, H, Err, FacRej applied / Hnew clipped to H, after first acc)]."""
+    ros_A = (0.0, 2.0, 0.0, 2.0, 0.0, 1.0)            # :443-452
+    ros_C = (4.0, 1.0, -1.0, 1.0, -1.0, -(8.0 / 3.0))
+    ros_M = (2.0, 0.0, 1.0, 1.0)
+    ros_E = (0.0, 0.0, 0.0, 1.0)
+    FacMin, FacMax, FacRej, FacSafe = 0.2, 6.0, 0.1, 0.9
+    ROUNDOFF = 2.0 ** -52
+    Hmin, Hmax = 0.0, Tend                            # :504-512, span = Tend - 0
+    H = min(abs(Hstart), Tend)
+    H = min(max(abs(Hmin), abs(H)), abs(Hmax))
+    T, y = 0.0, y0
+    ist = [0] * 8
+    rst = [0.0] * 3
+    rej_last = rej_more = False
+    attempts = []
+    while (T - Tend) + ROUNDOFF <= 0.0:               # :535
+        H = min(H, abs(Tend - T))
+        f0 = f(y); ist[0] += 1
+        j0 = J(y); ist[1] += 1
+        while True:
+            g = -j0 + 1.0 / (H * 0.5); ist[5] += 1
+            K = []
+            for s in range(4):
+                fs = f0 if s < 2 else None
+                if fs is None:                        # ros_NewF(3), ros_NewF(4)
+                    p = y
+                    for j in range(s):
+                        p = p + ros_A[s * (s - 1) // 2 + j] * K[j]
+                    fs = f(p); ist[0] += 1
+                r = fs
+                for j in range(s):
+                    r = r + (ros_C[s * (s - 1) // 2 + j] / H) * K[j]
+                K.append(r / g); ist[6] += 1
+            ynew = y
+            for j in range(4):
+                ynew = ynew + ros_M[j] * K[j]
+            yerr = 0.0
+            for j in range(4):
+                yerr = yerr + ros_E[j] * K[j]
+            q = yerr / (atol + rtol * max(abs(y), abs(ynew)))
+            err = max(np.sqrt(q * q / 1.0), 1.0e-10)
+            hnew = H * min(FacMax, max(FacMin, FacSafe / err ** (1.0 / 3.0)))
+            ist[2] += 1
+            if err <= 1.0:                            # accept, :676-692
+                ist[3] += 1
+                y = ynew
+                T = T + H
+                hnew = max(Hmin, min(hnew, Hmax))
+                clipped = rej_last and hnew > H
+                if rej_last:
+                    hnew = min(hnew, H)               # :685
+                rst = [T, H, hnew]
+                rej_last = rej_more = False
+                attempts.append((True, H, err, clipped, ist[3] > 1))
+                H = hnew
+                break
+            facrej = rej_more                         # reject, :693-698
+            if rej_more:
+                hnew = H * FacRej
+            rej_more = rej_last
+            rej_last = True
+            attempts.append((False, H, err, facrej, ist[3] >= 1))
+            H = hnew
+            if ist[3] >= 1:
+                ist[4] += 1                           # Nrej only after the first acceptance
+    return y, ist, rst, attempts
+
+
+def _quad1_case(c=QUAD1):
+    ir = synthetic.build_ir("quad1", 1, 1, [synthetic.Reaction({0: 2}, {}),
+                                            synthetic.Reaction({}, {0: 1.0}, {0: 1})],
+                            reorder=False)
+    C = np.array([[c["y0"], 1.0]])
+    R = np.array([[c["k"], c["p"]]])
+    I = np.zeros((1, 20), np.int32)
+    I[0, 0], I[0, 2] = 1, 4
+    Rc = np.zeros((1, 20))
+    Rc[0, 2] = c["Hstart"]
+    return ir, C, R, I, Rc
+
+
+def test_step_control_known_answer():
+    """2X -> (nothing) at k and F -> X at p (F = 1): Vdot = p - ((k*2)*y)*y,
+    J = -(2*((k*2)*y)) in KPP's generated forms.  With Hstart = 300 s
+    (clamped to the 100 s span) and rtol 1e-4 the sequence of attempts holds:
+    three rejections before the first acceptance, which Nrej does not count,
+    the third with H*FacRej; an acceptance right after rejections whose Hnew
+    is clipped to H; three counted rejections later, again ending in
+    H*FacRej and a clipped acceptance.  The oracle (libm pow, as the hand
+    version) gives the same ISTAT bit for bit and the same y and RSTAT."""
+    c = QUAD1
+    k, p = c["k"], c["p"]
+    y, ist, rst, att = rodas3_scalar_by_hand(
+        lambda v: p * 1.0 - ((k * 2.0) * v) * v, lambda v: -(2.0 * ((k * 2.0) * v)),
+        c["y0"], c["Tend"], c["Hstart"], c["atol"], c["rtol"])
+    first = next(n for n, a in enumerate(att) if a[0])
+    assert first == 3 and not any(a[0] for a in att[:3])
+    assert att[2][3] and not att[1][3]               # FacRej on the third rejection
+    assert att[first][3]                              # Hnew clipped to H
+    later = [a for a in att[first:] if not a[0]]
+    assert len(later) == 3 and later[-1][3]
+    assert ist[4] == 3 and ist[2] - ist[3] == 6
+    assert sum(1 for a in att if a[0] and a[3]) == 2
+    ir, C, R, I, Rc = _quad1_case()
+    for mode in ("libm", "cr"):
+        o = Oracle(ir, pow_mode=mode)
+        C1, IS, RS, IE, _ = o.integrate_batch(0.0, c["Tend"], C, R, np.array([c["atol"]]),
+                                              np.array([c["rtol"]]), I, Rc)
+        assert IE[0] == 1
+        assert list(IS[0]) == ist, mode
+        assert C1[0, 0] == y and list(RS[0]) == rst, mode
