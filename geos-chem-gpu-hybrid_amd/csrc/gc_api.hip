@@ -135,6 +135,14 @@ bool same_args(const GcBatchArgs& a, const GcBatchArgs& b) {
          a.IERR == b.IERR;
 }
 
+// Device scratch of gc_amd_integrate_batch_noalloc: the queue counter and
+// the launch-order flag in its first 4 KiB, then the persistent kernel's
+// workspace slots.  Every device holds its own copy until the process ends;
+// host code uses its per-device address (DevCtx::scratch), never this name.
+constexpr size_t kNoAllocBytes = size_t(256) << 20;
+constexpr size_t kNoAllocHead = 4096;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 struct DevCtx {
   std::mutex mu;
   bool init = false;
@@ -142,6 +150,8 @@ struct DevCtx {
   int n_cu = 0;
   DevBuf ws, queue, flag;
   ChunkGraph graph;
+  unsigned char* scratch = nullptr;  // this device's copy of `scratch`
+  hipError_t scratch_err = hipSuccess;
   // staging for the host-array entry points
   DevBuf C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -161,6 +171,12 @@ hipError_t ctx_init(DevCtx& c, int dev) {
   hipDeviceProp_t prop;
   if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) return e;
   c.n_cu = prop.multiProcessorCount;
+  // the address of this device's scratch array (the allocation-free entry
+  // point's workspace); the other entry points do not need it
+  void* sp = nullptr;
+  c.scratch_err = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+  c.scratch = c.scratch_err == hipSuccess ? (unsigned char*)sp : nullptr;
+  (void)hipGetLastError();
   c.init = true;
   return hipSuccess;
 }
@@ -194,14 +210,12 @@ __global__ void k_zero_u32(unsigned* p, int n) {
 // ---- the device solve ----------------------------------------------------------
 // Launches one chunk (a contiguous range of the launch order) and returns the
 // runtime status of the launch.
-hipError_t launch_chunk(DevCtx& c, int kernel, const Tuning& t, const GcBatchArgs& b,
-                        unsigned S, hipStream_t st) {
-  double* ws = (double*)c.ws.p;
+hipError_t launch_chunk(int kernel, const Tuning& t, const GcBatchArgs& b, unsigned S,
+                        double* ws, unsigned* q, hipStream_t st) {
   if (kernel == GC_AMD_KERNEL_PLAIN) {
     // the reference's launch: one thread per cell, 128-thread work-groups
     return gc_launch_plain_v1(&b, ws, 128, st);
   }
-  unsigned* q = (unsigned*)c.queue.p;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, q, 1);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -219,8 +233,9 @@ hipError_t launch_chunk(DevCtx& c, int kernel, const Tuning& t, const GcBatchArg
   return gc_launch_persistent_v1(&b, ws, S, q, t.block, t.cells_per_wave, grain, st);
 }
 
-// Number of slots (lanes with workspace) for a launch of n cells.
-unsigned slots_for(DevCtx& c, int kernel, const Tuning& t, int n) {
+// Number of slots (lanes with workspace) for a launch of n cells; a
+// persistent grid has at most max_blocks work-groups when that is > 0.
+unsigned slots_for(DevCtx& c, int kernel, const Tuning& t, int n, unsigned max_blocks = 0) {
   if (kernel == GC_AMD_KERNEL_PLAIN) return ((unsigned)n + 63u) & ~63u;
   // persistent: one resident grid; no more waves than there are cells to hand out
   int bpc = t.blocks_per_cu;
@@ -229,7 +244,8 @@ unsigned slots_for(DevCtx& c, int kernel, const Tuning& t, int n) {
     bpc = waves_per_cu / (t.block / 64);
     if (bpc < 1) bpc = 1;
   }
-  const unsigned grid_cap = (unsigned)(c.n_cu * bpc);
+  unsigned grid_cap = (unsigned)(c.n_cu * bpc);
+  if (max_blocks > 0 && max_blocks < grid_cap) grid_cap = max_blocks;
   const unsigned cells_per_block = (unsigned)(t.block / 64 * t.cells_per_wave);
   unsigned need = ((unsigned)n + cells_per_block - 1u) / cells_per_block;
   if (need < 1u) need = 1u;
@@ -239,14 +255,19 @@ unsigned slots_for(DevCtx& c, int kernel, const Tuning& t, int n) {
 
 // Launch order must be a map into 1..nCell (a bad entry would address outside
 // the caller's arrays).  Returns 0, -1 (bad entry), -2 or -3.
-int check_perm(DevCtx& c, const int* Perm, int nCell, hipStream_t st) {
+// `flag` is one int of device memory (the library's own when null).
+int check_perm(DevCtx& c, const int* Perm, int nCell, hipStream_t st, int* flag = nullptr) {
   hipError_t e;
-  if ((e = c.flag.ensure(sizeof(int))) != hipSuccess) { set_error("hipMalloc(flag)", e); return -2; }
-  if ((e = hipMemsetAsync(c.flag.p, 0, sizeof(int), st)) != hipSuccess) { set_error("hipMemsetAsync", e); return -3; }
+  if (!flag) {
+    if ((e = c.flag.ensure(sizeof(int))) != hipSuccess) { set_error("hipMalloc(flag)", e); return -2; }
+    flag = (int*)c.flag.p;
+  }
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, (unsigned*)flag, 1);
   hipLaunchKernelGGL(k_check_perm, dim3((nCell + 255) / 256), dim3(256), 0, st, Perm, nCell,
-                     (int*)c.flag.p);
+                     flag);
   int bad = 0;
-  if ((e = hipMemcpyAsync(&bad, c.flag.p, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess ||
       (e = hipStreamSynchronize(st)) != hipSuccess) {
     set_error("perm check", e);
     return -3;
@@ -285,7 +306,7 @@ int device_solve(DevCtx& c, int kernel, int chunk, const GcBatchArgs& b, hipStre
 
   if (span == b.nCell) {
     if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
-    e = launch_chunk(c, kernel, t, b, S, st);
+    e = launch_chunk(kernel, t, b, S, (double*)c.ws.p, (unsigned*)c.queue.p, st);
     if (e != hipSuccess) { set_error("launch", e); return -3; }
   } else {
     // The chunk loop over the launch order as one graph: captured and
@@ -304,7 +325,7 @@ int device_solve(DevCtx& c, int kernel, int chunk, const GcBatchArgs& b, hipStre
         GcBatchArgs sub = b;
         sub.Perm = b.Perm + c0;
         sub.nCell = (b.nCell - c0 < span) ? (b.nCell - c0) : span;
-        le = launch_chunk(c, kernel, t, sub, S, st);
+        le = launch_chunk(kernel, t, sub, S, (double*)c.ws.p, (unsigned*)c.queue.p, st);
       }
       e = hipStreamEndCapture(st, &g);
       if (le != hipSuccess || e != hipSuccess) {
@@ -696,6 +717,73 @@ int gc_amd_integrate_batch_device(int kernel, int chunk, int nCell, double Tstar
     return -3;
   }
   return device_solve(*cp, kernel, chunk, b, cp->stream, tKernel);
+}
+
+int gc_amd_integrate_batch_noalloc(int kernel, int nCell, double Tstart, double Tend, double* C,
+                                   const double* RCONST, const double* ATOL, const double* RTOL,
+                                   const int* ICNTRL, const double* RCNTRL, const int* Perm,
+                                   int* ISTAT, double* RSTAT, int* IERR, void* stream,
+                                   double* tKernel) {
+  if (tKernel) *tKernel = 0.0;
+  if (nCell < 0 || (kernel != GC_AMD_KERNEL_PLAIN && kernel != GC_AMD_KERNEL_PERSISTENT)) {
+    set_error("gc_amd_integrate_batch_noalloc: bad argument");
+    return -1;
+  }
+  const long wave_bytes = ws_per_slot() * 8L * 64L;
+  const unsigned max_waves = (unsigned)((kNoAllocBytes - kNoAllocHead) / (size_t)wave_bytes);
+  if (kernel == GC_AMD_KERNEL_PLAIN && (unsigned)nCell > max_waves * 64u) {
+    set_error("gc_amd_integrate_batch_noalloc: nCell exceeds the cells the plain kernel can "
+              "solve in the 256 MiB scratch (one workspace slot per cell)");
+    return -1;
+  }
+  if (max_waves < 1u) {
+    set_error("gc_amd_integrate_batch_noalloc: one wave's workspace exceeds the scratch");
+    return -1;
+  }
+  int rc = 0;
+  DevCtx* cp = enter(stream, &rc);
+  if (!cp) return rc;
+  DevCtx& c = *cp;
+  if (!c.scratch) {
+    set_error("gc_amd_integrate_batch_noalloc: scratch address", c.scratch_err);
+    c.mu.unlock();
+    return -3;
+  }
+  if (nCell == 0) {
+    c.mu.unlock();
+    return 0;
+  }
+  hipStream_t st = c.stream;
+  unsigned* q = (unsigned*)c.scratch;
+  int* flag = (int*)(c.scratch + 256);
+  double* ws = (double*)(c.scratch + kNoAllocHead);
+  rc = check_perm(c, Perm, nCell, st, flag);
+  if (rc != 0) {
+    c.mu.unlock();
+    return rc;
+  }
+  Tuning t = current_tuning();
+  unsigned S = 0;
+  if (kernel == GC_AMD_KERNEL_PLAIN) {
+    S = slots_for(c, kernel, t, nCell);
+  } else {
+    // a resident grid of whole work-groups inside the scratch's slots
+    if ((unsigned)(t.block / 64) > max_waves) t.block = 64 * (int)max_waves;
+    S = slots_for(c, kernel, t, nCell, max_waves / (unsigned)(t.block / 64));
+  }
+  GcBatchArgs b{nCell, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR};
+  hipError_t e;
+  if ((e = hipEventRecord(c.ev0, st)) != hipSuccess ||
+      (e = launch_chunk(kernel, t, b, S, ws, q, st)) != hipSuccess ||
+      (e = hipEventRecord(c.ev1, st)) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess) {
+    set_error("gc_amd_integrate_batch_noalloc: solve", e);
+    c.mu.unlock();
+    return -3;
+  }
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess && tKernel) *tKernel = ms * 1e-3;
+  c.mu.unlock();
+  return 0;
 }
 
 int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave, int grain, int variant) {

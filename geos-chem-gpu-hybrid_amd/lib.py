@@ -51,6 +51,7 @@ SIGNATURES = {
     "gc_gpu_set_device": (I, [I]),
     "gc_gpu_selftest": (None, [IP, IP, DP]),
     "gc_amd_integrate_batch_device": (I, [I, I, I, D, D, P, P, P, P, P, P, P, P, P, P, P, DP]),
+    "gc_amd_integrate_batch_noalloc": (I, [I, I, D, D, P, P, P, P, P, P, P, P, P, P, P, DP]),
     "gc_amd_record_batch_device": (I, [I, D, D] + [P] * 14 + [ctypes.c_long, P, ctypes.c_long, I,
                                        P, ctypes.POINTER(ctypes.c_long),
                                        ctypes.POINTER(ctypes.c_long), P, DP]),

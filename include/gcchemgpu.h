@@ -102,6 +102,37 @@ int gc_amd_integrate_batch_device(int kernel, int chunk, int nCell,
                                   int *ISTAT, double *RSTAT, int *IERR,
                                   void *stream, double *tKernel);
 
+/* Allocation-free form of gc_amd_integrate_batch_device (chunk = 0): the
+ * same arguments, results and return codes, with every device buffer the
+ * solve needs beyond the caller's inputs and outputs -- the work-queue
+ * counter, the launch-order check flag and the workspace slots -- taken from
+ * a 256 MiB scratch array that each device holds from the library's first
+ * call on it until the process ends.  The call itself allocates nothing (no
+ * hipMalloc, hipFree or allocating library routine); the first call of the
+ * library on a device, of any entry point, sets up the device's stream and
+ * events and finds its scratch address.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).
+ * Limits: the persistent kernel solves any nCell with the resident grid the
+ * scratch holds (floor((256 MiB - 4 KiB) / (512 * W)) waves of 64 slots,
+ * W = gc_amd_workspace_doubles_per_slot(): 40 waves for fullchem), so it is
+ * far slower than gc_amd_integrate_batch_device on large batches; the plain
+ * kernel needs one slot per cell and returns -1 when nCell exceeds
+ * 64 x that many waves (2,560 cells for fullchem).  Every call on a device
+ * uses the same scratch, which keeps the last call's contents (each call
+ * clears the counter and the flag it needs); two calls that overlapped on one
+ * device would race on it, so the library serialises the calls of a device:
+ * a call made while another is running on the same device waits for it.
+ * Returns 0, -1 (bad argument, Perm entry outside 1..nCell, nCell beyond the
+ * plain kernel's limit), -3 (runtime error; gc_amd_last_error).  Blocking;
+ * ordered after `stream`. */
+int gc_amd_integrate_batch_noalloc(int kernel, int nCell, double Tstart,
+                                   double Tend, double *C, const double *RCONST,
+                                   const double *ATOL, const double *RTOL,
+                                   const int *ICNTRL, const double *RCNTRL,
+                                   const int *Perm, int *ISTAT, double *RSTAT,
+                                   int *IERR, void *stream, double *tKernel);
+
 /* Launch tuning of the persistent kernel (values <= 0 keep the current one):
  *   block           work-group size (64..256, multiple of 64)
  *   blocks_per_cu   resident work-groups per compute unit (grid = CUs * this)
