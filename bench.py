@@ -180,6 +180,27 @@ def host_api_rate(lib, d, perm):
     return n / tT.value, tT.value, tK.value
 
 
+def resident_rate(d):
+    """Two chemistry steps of a device-resident batch (gc_amd_resident_*):
+    the first stands for the previous step (identity order, untimed), the
+    second is a model step -- C and RCONST uploaded from host arrays through
+    the pinned staging, the launch order computed on the device from the
+    first step's step counts, C, ISTAT, RSTAT and IERR copied back.  Cold
+    start on both (warm = 0), so the solve is the timed persistent one:
+    (cells/s, tTotal, tKernel) of the second step."""
+    from gcchem_amd import resident
+    n = d["C"].shape[0]
+    r = resident.Resident(d["ICNTRL"], d["RCNTRL"], d["ATOL"], d["RTOL"])
+    try:
+        r.step(d["Tstart"], d["Tend"], C=d["C"], RCONST=d["RCONST"], warm=False, outputs=False)
+        out = r.step(d["Tstart"], d["Tend"], C=d["C"], RCONST=d["RCONST"], warm=False)
+        if not (out[3] == 1).all():
+            raise RuntimeError(f"resident step: {int((out[3] != 1).sum())} cells failed")
+        return n / r.t_total, r.t_total, r.t_kernel
+    finally:
+        r.close()
+
+
 def membench(block_doubles, waves_per_cu, loads_per_store):
     """Memory ceiling of the workspace access shape on this device
     (csrc/gc_membench.hip, built by build.py): random 512-byte wave accesses
@@ -261,73 +282,75 @@ STRESS_SIZES = (1000, 4000, 16000, 32000, 64000, 100000, 150000, 200000)
 
 def bench_stress(args):
     """Synthetic stress mechanism (500 species, 1,500 reactions, ~3% LU fill)
-    in eight ragged batches of 1k-200k cells, dt 600 s, launch order = batch
-    order (no previous step to sort by): each batch with the persistent and
-    the plain kernel, then all batches in one persistent launch when the
-    per-batch timings say it finishes in under 100 s, and the adjoint
-    (record + sweep, loss gradient dL/dC_final = 1) of the first batch."""
+    in eight ragged batches of 1k-200k cells, dt 600 s, each batch in its
+    own launch order (no previous step to sort by): all batches in ONE
+    persistent launch (gc_amd_integrate_batches_device, one work queue
+    through the batches, progress lines as cells finish), the plain kernel
+    on each batch in turn for the ratio, and the adjoint (dL/dC_final = 1) of
+    the two smallest batches as one recording launch plus one sweep over both,
+    against the forward launch of the same two batches."""
     import numpy as np
     import torch
     from gcchem_amd import adjoint, inputs, mechanism, solver
     mech = mechanism.load("stress500")
     sizes = [int(x) for x in args.stress_sizes.split(",")] if args.stress_sizes else STRESS_SIZES
-    per, total_p, total_q, adj = [], 0.0, 0.0, []
-    batches = []
-    for i, n in enumerate(sizes):
-        d = inputs.stress_cells(mech, n, seed=40 + i)
-        b = solver.Batch.from_numpy(d)
-        rp = solver.solve(b, kernel="persistent", mech="stress500")
-        torch.cuda.synchronize()
-        ok = float((rp.IERR == 1).float().mean().item())
-        nst = float(rp.ISTAT[:, 2].float().mean())
-        log(f"stress batch {n}: persistent {rp.t_kernel:.2f} s, mean steps {nst:.0f}, "
-            f"IERR=1 fraction {ok:.4f} (the rest -6, for the caller's CPU retry)")
+    bs = [solver.Batch.from_numpy(inputs.stress_cells(mech, n, seed=40 + i))
+          for i, n in enumerate(sizes)]
+    ncell = sum(sizes)
+    log(f"stress: {len(sizes)} batches {sizes} generated ({ncell} cells)")
+    step = [0.05]
+
+    def progress(nb, done, total):
+        if done >= step[0] * total or nb == len(sizes):
+            log(f"one launch: {done}/{total} cells finished, {nb}/{len(sizes)} batches")
+            while step[0] * total <= done and step[0] <= 1.0:
+                step[0] += 0.05
+
+    outs, t_one = solver.solve_batches(bs, mech="stress500", progress=progress)
+    torch.cuda.synchronize()
+    log(f"all {len(sizes)} batches in one persistent launch: {t_one:.2f} s")
+    per, total_q = [], 0.0
+    for b, r, n in zip(bs, outs, sizes):
+        ok = float((r.IERR == 1).float().mean().item())
+        nst = float(r.ISTAT[:, 2].float().mean())
         rq = solver.solve(b, kernel="plain", mech="stress500")
         torch.cuda.synchronize()
-        log(f"stress batch {n}: plain {rq.t_kernel:.2f} s")
-        per.append({"cells": n, "persistent_s": round(rp.t_kernel, 3), "plain_s": round(rq.t_kernel, 3),
-                    "mean_steps": round(nst, 1), "ierr_ok_frac": ok})
-        total_p += rp.t_kernel
+        same = all(torch.equal(x, y) for x, y in ((r.C, rq.C), (r.ISTAT, rq.ISTAT),
+                                                  (r.RSTAT, rq.RSTAT), (r.IERR, rq.IERR)))
+        log(f"stress batch {n}: plain {rq.t_kernel:.2f} s, mean steps {nst:.0f}, IERR=1 "
+            f"fraction {ok:.4f} (the rest -6, for the caller's CPU retry), one launch == "
+            f"plain: {same}")
+        per.append({"cells": n, "plain_s": round(rq.t_kernel, 3), "mean_steps": round(nst, 1),
+                    "ierr_ok_frac": ok, "one_launch_equals_plain": same})
         total_q += rq.t_kernel
-        if i == 0 and not args.no_adjoint:
-            out, rec = adjoint.record_solve(b, mech="stress500")
-            torch.cuda.synchronize()
-            log(f"stress batch {n}: recording forward {rec.t_kernel:.2f} s")
-            lam = torch.ones_like(out.C)
-            g, gr, ierr, t_sweep = adjoint.adjoint_sweep(b, rec, lam, mech="stress500")
-            torch.cuda.synchronize()
-            ratio = (rec.t_kernel + t_sweep) / rp.t_kernel
-            log(f"stress batch {n}: adjoint record {rec.t_kernel:.2f} s + sweep {t_sweep:.2f} s "
-                f"(checkpoint every {rec.every}), adj/fwd {ratio:.2f}")
-            adj.append({"cells": n, "record_s": round(rec.t_kernel, 3), "sweep_s": round(t_sweep, 3),
-                        "every": rec.every, "adj_over_fwd": round(ratio, 3),
-                        "ierr_ok_frac": float((ierr == 1).float().mean().item())})
-        batches.append(d)
-        del b
-    ncell = sum(sizes)
-    one = None
-    if total_p < 100.0:
-        cat = {k: np.concatenate([d[k] for d in batches]) for k in ("C", "RCONST", "ICNTRL", "RCNTRL")}
-        cat.update({k: batches[0][k] for k in ("ATOL", "RTOL", "Tstart", "Tend")})
-        cat["Perm"] = np.arange(1, ncell + 1, dtype=np.int32)
-        b = solver.Batch.from_numpy(cat)
-        r = solver.solve(b, kernel="persistent", mech="stress500")
+        del rq
+    adj = None
+    if not args.no_adjoint:
+        two = bs[:2]
+        _, t_fwd = solver.solve_batches(two, mech="stress500")
+        outs2, rec = adjoint.record_solve_batches(two, mech="stress500")
         torch.cuda.synchronize()
-        one = round(r.t_kernel, 3)
-        log(f"all {len(sizes)} batches in one persistent launch: {r.t_kernel:.2f} s")
-    t = one if one is not None else total_p
+        log(f"adjoint, batches {sizes[:2]}: forward {t_fwd:.2f} s, recording {rec.t_kernel:.2f} s")
+        lams = [torch.ones_like(o.C) for o in outs2]
+        grads, t_sweep = adjoint.adjoint_sweep_batches(two, rec, lams, mech="stress500")
+        torch.cuda.synchronize()
+        ratio = (rec.t_kernel + t_sweep) / t_fwd
+        log(f"adjoint, batches {sizes[:2]}: sweep {t_sweep:.2f} s, adj/fwd {ratio:.2f}")
+        adj = {"batches": sizes[:2], "forward_s": round(t_fwd, 3), "record_s": round(rec.t_kernel, 3),
+               "sweep_s": round(t_sweep, 3), "records": rec.used, "adj_over_fwd": round(ratio, 3),
+               "ierr_ok_frac": float(np.mean([float((g[2] == 1).float().mean()) for g in grads]))}
     return {
-        "metric": METRIC, "value": round(ncell / t, 1), "unit": "cells/s", "n_gpus": 1,
-        "steps": 1, "warmup": 0, "ms_per_step": round(t * 1e3, 2), "higher_is_better": True,
+        "metric": METRIC, "value": round(ncell / t_one, 1), "unit": "cells/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": round(t_one * 1e3, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"synthetic stress mechanism 500 species x 1500 reactions, "
-                               f"{len(sizes)} ragged batches {sizes}, dt 600 s, "
-                               f"Max_no_steps 1000 (longer cells -> IERR -6, CPU retry)",
-                   "cells": ncell, "one_launch": one is not None},
+                               f"{len(sizes)} ragged batches {sizes} in one persistent launch, "
+                               f"dt 600 s, Max_no_steps 1000 (longer cells -> IERR -6, CPU retry)",
+                   "cells": ncell, "one_launch": True},
         "plain_cells_per_s": round(ncell / total_q, 1),
-        "speedup_vs_plain": round(total_q / t, 3),
+        "speedup_vs_plain": round(total_q / t_one, 3),
         "batches": per, "adjoint": adj,
-        "adj_over_fwd": adj[-1]["adj_over_fwd"] if adj else None,
+        "adj_over_fwd": adj["adj_over_fwd"] if adj else None,
         "roofline": None, "cpu_baseline": None,
     }
 
@@ -414,6 +437,8 @@ def main():
     b.Perm = torch.as_tensor(perm, device=b.C.device)
 
     host_rate, host_tt, host_tk = host_api_rate(lib, d, perm)
+    res_rate, res_tt, res_tk = resident_rate(d)
+    log(f"resident step: tTotal {res_tt * 1e3:.1f} ms, tKernel {res_tk * 1e3:.1f} ms")
     adj = adjoint_ratio(mech) if not args.no_adjoint else None
     log(f"host-array entry point: tTotal {host_tt * 1e3:.1f} ms, tKernel {host_tk * 1e3:.1f} ms")
 
@@ -461,6 +486,10 @@ def main():
         "s_per_chem_step": round(s_pers, 4),
         "host_api_cells_per_s": round(host_rate, 1),
         "host_api_t_total_s": round(host_tt, 4),
+        "host_api_frac_of_value": round(host_rate / (n / s_pers), 4),
+        "resident_step": {"cells_per_s": round(res_rate, 1), "t_total_s": round(res_tt, 4),
+                          "t_kernel_s": round(res_tk, 4),
+                          "frac_of_value": round(res_rate / (n / s_pers), 4)},
         "adj_over_fwd": adj["adj_over_fwd"] if adj else None,
         "adjoint": adj,
         "ierr_ok_frac": float((ierr == 1).mean()),

@@ -177,3 +177,85 @@ def solve(C: torch.Tensor, RCONST: torch.Tensor, template: Batch,
     """Differentiable batch solve: C [n, NSPEC], RCONST [n, NREACT] (device
     tensors, may require grad); the other inputs come from `template`."""
     return Rodas3Solve.apply(C, RCONST, template, mech)
+
+
+def record_solve_batches(batches: list[Batch], mech: str = "fullchem", every: int = 0,
+                         cap: int = 0, state_budget: float = STATE_BUDGET_BYTES,
+                         stage_budget: float = STAGE_BUDGET_BYTES, progress=None):
+    """Recording forward solve of several batches in one persistent launch
+    (gc_amd_record_batches_device): their results and one Records whose head
+    covers the batches' cells one after the other.  every, cap and the
+    budgets as in record_solve (pools too small are re-sized to the exact
+    need and the launch repeated); progress(batches_done, cells_done,
+    cells_total) as in solver.solve_batches."""
+    from .solver import _desc, _results
+    lib = _lib.load(mech)
+    nvar = _lib.mech_info(lib)[0]
+    for b in batches:
+        _check(b, _lib.mech_info(lib))
+    n = sum(b.n for b in batches)
+    dev = batches[0].C.device
+    state_bytes = nvar * 8
+    cap = int(cap or max(1, n * DEFAULT_STEPS_PER_CELL))
+    ev = int(every) if every else max(1, -(-cap * state_bytes // int(state_budget)))
+    cb = _lib.PROGRESS_FN(lambda nb, c, t, u: progress(nb, c, t)) if progress else None
+    for _ in range(3):
+        ycap = max(1, min(cap, -(-cap // ev) + n))
+        outs = _results(batches)
+        recH = torch.empty((cap,), dtype=torch.float64, device=dev)
+        prev = torch.empty((cap,), dtype=torch.int32, device=dev)
+        yslot = torch.empty((cap,), dtype=torch.int32, device=dev)
+        head = torch.empty((n,), dtype=torch.int32, device=dev)
+        recY = torch.empty((ycap, nvar), dtype=torch.float64, device=dev)
+        recK = None
+        if ev == 1 and ycap * 4 * state_bytes <= stage_budget:
+            recK = torch.empty((ycap, 4, nvar), dtype=torch.float64, device=dev)
+        descs = (_lib.GcAmdBatchDesc * len(batches))(*[_desc(b, r) for b, r in zip(batches, outs)])
+        used, yused, tk = ctypes.c_long(0), ctypes.c_long(0), ctypes.c_double(0.0)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        rc = lib.gc_amd_record_batches_device(
+            len(batches), descs, _p(recH), _p(prev), _p(yslot), _p(head), cap, _p(recY), ycap,
+            ev, None if recK is None else _p(recK), ctypes.byref(used), ctypes.byref(yused),
+            cb, None, ctypes.c_void_p(stream), ctypes.byref(tk))
+        if rc == -4:
+            del recH, prev, yslot, recY, recK
+            cap = max(cap, int(used.value))
+            if not every:
+                ev = max(ev, -(-int(used.value) * state_bytes // int(state_budget)))
+            continue
+        if rc != 0:
+            raise RuntimeError(f"gc_amd_record_batches_device returned {rc}: {_lib.last_error(lib)}")
+        for r in outs:
+            r.t_kernel = tk.value
+        return outs, Records(recH, prev, yslot, head, recY, recK, ev, int(used.value),
+                             int(yused.value), tk.value)
+    raise RuntimeError("record pool sizing failed")
+
+
+def adjoint_sweep_batches(batches: list[Batch], rec: Records, lams: list[torch.Tensor],
+                          mech: str = "fullchem", progress=None):
+    """Reverse sweep of a record_solve_batches recording in one launch: per
+    batch (dL/dC0, dL/dRCONST, IERR), and the kernel seconds; progress as in
+    record_solve_batches."""
+    lib = _lib.load(mech)
+    outs, descs = [], []
+    for b, lam in zip(batches, lams):
+        lam = lam.to(device=b.C.device, dtype=torch.float64).contiguous()
+        gC, gR = torch.empty_like(b.C), torch.empty_like(b.RCONST)
+        ierr = torch.empty((b.n,), dtype=torch.int32, device=b.C.device)
+        outs.append((gC, gR, ierr, lam))
+        p = lambda t: t.data_ptr()
+        descs.append(_lib.GcAmdAdjBatch(b.n, float(b.Tstart), float(b.Tend), p(b.C), p(b.RCONST),
+                                        p(b.ICNTRL), p(b.Perm), p(lam), p(gC), p(gR), p(ierr)))
+    arr = (_lib.GcAmdAdjBatch * len(descs))(*descs)
+    cb = _lib.PROGRESS_FN(lambda nb, c, t, u: progress(nb, c, t)) if progress else None
+    tk = ctypes.c_double(0.0)
+    stream = torch.cuda.current_stream(batches[0].C.device).cuda_stream
+    rc = lib.gc_amd_adjoint_batches_device(
+        len(descs), arr, _p(rec.H), _p(rec.prev), _p(rec.yslot), _p(rec.head),
+        int(rec.H.shape[0]), _p(rec.Y), int(rec.Y.shape[0]), int(rec.every),
+        None if rec.K is None else _p(rec.K), cb, None, ctypes.c_void_p(stream),
+        ctypes.byref(tk))
+    if rc != 0:
+        raise RuntimeError(f"gc_amd_adjoint_batches_device returned {rc}: {_lib.last_error(lib)}")
+    return [o[:3] for o in outs], tk.value

@@ -85,6 +85,11 @@ from pathlib import Path
 SEGMENTS = (16, 24, 32)  # LU row-segment sizes compiled (LDS entries per lane)
 
 
+def mask_words(n: int) -> int:
+    """32-bit words of a mask of n flags (flag k: bit k % 32 of word k // 32)."""
+    return (n + 31) // 32
+
+
 def layout(ir) -> dict:
     nvar, nfix, nreact, nnz, nb = ir["nvar"], ir["nfix"], ir["nreact"], ir["nnz"], ir["nb"]
     E = {}
@@ -102,7 +107,12 @@ def layout(ir) -> dict:
     E["ZERO"] = E["LU"] + nnz  # holds 0.0 (padding operand)
     E["SINK"] = E["ZERO"] + 1  # written by padding stores, never read
     E["ONE"] = E["SINK"] + 1   # holds 1.0 (missing factor of a D_VAR term)
-    E["TOTAL"] = E["ONE"] + 1
+    # auto-reduce masks of the cell (masked kernels only): 32 flags per
+    # element, in its low 32 bits -- DO_FUN(NVAR), DO_JVS(LU_NONZERO), DO_SLV(NVAR)
+    E["MFUN"] = E["ONE"] + 1
+    E["MJVS"] = E["MFUN"] + mask_words(nvar)
+    E["MSLV"] = E["MJVS"] + mask_words(nnz)
+    E["TOTAL"] = E["MSLV"] + mask_words(nvar)
     # discrete adjoint: the forward block followed by the adjoint vectors
     E["LAM"] = E["TOTAL"]          # lambda = dL/dY at the end of the step
     E["LK"] = E["LAM"] + nvar      # 4 stage adjoints lambda_K
@@ -231,15 +241,20 @@ class Tables:
         """All JVS entries of one row segment as one ENTRYG record {5, T, 0, 0}
         + T terms {soff, flags, coef_lo, coef_hi} (T a multiple of 4), the
         kernel's loads going out eight or four terms at a time.  flags: 1 first
-        term of an entry, 2 last term, 4 diagonal entry, slot index << 8.  An
-        entry with no term (a structural zero of the LU pattern) is one term
+        term of an entry, 2 last term, 4 diagonal entry, slot index << 8 (6
+        bits), the entry's JVS index k << 14 (for its DO_JVS flag).  An entry
+        with no term (a structural zero of the LU pattern) is one term
         1.0*ZERO, whose +0.0 is the empty sum; padding terms write the spare
         slot."""
+        if self.ir["nnz"] >= 1 << 18:
+            raise ValueError("JVS index beyond the 18 bits of an entry term's flags")
         resolve = lambda x: self.data_off(x, "Y")
         zero = self.off("ZERO", 0)
         words = []
-        for sl, dg, terms in entries:
-            flags = (4 if dg else 0) | ((sl // 512) << 8)
+        for sl, dg, terms, k in entries:
+            if sl // 512 >= 64:
+                raise ValueError("LDS slot beyond the 6 bits of an entry term's flags")
+            flags = (4 if dg else 0) | ((sl // 512) << 8) | (k << 14)
             if not terms:
                 words += [zero, flags | 3, *_coef_words(1.0)]
                 continue
@@ -274,8 +289,8 @@ class Tables:
                 segcols = cols[s0:s0 + seg]
                 slot = {c: n * 512 for n, c in enumerate(segcols)}
                 lo = segcols[0]
-                ops += self.entry_group([(slot[c], c == k, jvs[pos[c]][2]) for c in segcols],
-                                        spare)
+                ops += self.entry_group([(slot[c], c == k, jvs[pos[c]][2], pos[c])
+                                         for c in segcols], spare)
                 for kk in range(crow[k], diag[k]):
                     j = icol[kk]
                     if j > segcols[-1]:
@@ -663,6 +678,9 @@ def generate(ir: dict) -> str:
         f"#define MECH_NFIX {ir['nfix']}",
         f"#define MECH_NREACT {ir['nreact']}",
         f"#define MECH_NNZ {ir['nnz']}",
+        f"#define MECH_MW_FUN {mask_words(ir['nvar'])}",
+        f"#define MECH_MW_JVS {mask_words(ir['nnz'])}",
+        f"#define MECH_MW_SLV {mask_words(ir['nvar'])}",
     ]
     for k, v in E.items():
         parts.append(f"#define MECH_E_{k} {v}")

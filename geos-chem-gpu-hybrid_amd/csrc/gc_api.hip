@@ -13,10 +13,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include "../../include/gcchemgpu.h"
 #include "gc_rodas3_kernels.h"
@@ -106,6 +110,138 @@ struct DevBuf {
   }
 };
 
+// ---- pinned staging of host arrays --------------------------------------------
+// Host arrays move through pinned buffers in chunks: two host threads copy
+// chunks into (or out of) their own pair of pinned buffers while the DMA
+// engine moves the chunk before, so the host copy and the transfer overlap
+// and the transfer runs at the pinned rate.  A device without pinned buffers
+// (hipHostMalloc failed) falls back to plain pageable copies.
+struct Piece {
+  void* dev;
+  void* host;
+  size_t bytes;
+};
+
+struct Stager {
+  static constexpr int kThreads = 2, kSlots = 2;
+  static constexpr size_t kChunk = size_t(32) << 20;
+  bool tried = false, ok = false;
+  unsigned char* pin[kThreads][kSlots] = {};
+  hipStream_t st[kThreads] = {};
+  hipEvent_t ev[kThreads][kSlots] = {};
+  hipEvent_t done[kThreads] = {};
+
+  void init() {
+    if (tried) return;
+    tried = true;
+    for (int t = 0; t < kThreads; ++t) {
+      if (hipStreamCreateWithFlags(&st[t], hipStreamNonBlocking) != hipSuccess) return;
+      if (hipEventCreateWithFlags(&done[t], hipEventDisableTiming) != hipSuccess) return;
+      for (int k = 0; k < kSlots; ++k) {
+        if (hipHostMalloc((void**)&pin[t][k], kChunk, hipHostMallocDefault) != hipSuccess) {
+          pin[t][k] = nullptr;
+          (void)hipGetLastError();
+          return;
+        }
+        if (hipEventCreateWithFlags(&ev[t][k], hipEventDisableTiming) != hipSuccess) return;
+      }
+    }
+    ok = true;
+  }
+
+  // chunk c of the pieces -> (piece, offset, length)
+  static std::vector<Piece> chunks(const Piece* p, int n) {
+    std::vector<Piece> out;
+    for (int i = 0; i < n; ++i)
+      for (size_t o = 0; o < p[i].bytes; o += kChunk)
+        out.push_back({(char*)p[i].dev + o, (char*)p[i].host + o, std::min(kChunk, p[i].bytes - o)});
+    return out;
+  }
+
+  // Host -> device, ordered after `after` (the copies start once it reaches
+  // this point) and before the work queued later on `consumer`.
+  hipError_t to_device(int dev, const Piece* p, int n, hipStream_t after, hipStream_t consumer) {
+    if (!ok) {
+      for (int i = 0; i < n; ++i) {
+        hipError_t e = hipMemcpyAsync(p[i].dev, p[i].host, p[i].bytes, hipMemcpyHostToDevice,
+                                      consumer);
+        if (e != hipSuccess) return e;
+      }
+      return hipSuccess;
+    }
+    const std::vector<Piece> c = chunks(p, n);
+    hipError_t err[kThreads] = {hipSuccess, hipSuccess};
+    if (hipEventRecord(done[0], after) != hipSuccess) return hipErrorUnknown;
+    for (int t = 0; t < kThreads; ++t)
+      if (hipStreamWaitEvent(st[t], done[0], 0) != hipSuccess) return hipErrorUnknown;
+    auto work = [&](int t) {
+      hipError_t e = hipSetDevice(dev);
+      for (size_t k = t, j = 0; k < c.size() && e == hipSuccess; k += kThreads, ++j) {
+        const int s = (int)(j % kSlots);
+        e = hipEventSynchronize(ev[t][s]);  // the slot's previous transfer is done
+        if (e != hipSuccess) break;
+        memcpy(pin[t][s], c[k].host, c[k].bytes);
+        e = hipMemcpyAsync(c[k].dev, pin[t][s], c[k].bytes, hipMemcpyHostToDevice, st[t]);
+        if (e == hipSuccess) e = hipEventRecord(ev[t][s], st[t]);
+      }
+      err[t] = e;
+    };
+    std::thread helper(work, 1);
+    work(0);
+    helper.join();
+    for (int t = 0; t < kThreads; ++t) {
+      if (err[t] != hipSuccess) return err[t];
+      hipError_t e = hipEventRecord(done[t], st[t]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(consumer, done[t], 0);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+
+  // Device -> host after the work queued on `producer`; returns when the
+  // host arrays hold the data.
+  hipError_t to_host(int dev, const Piece* p, int n, hipStream_t producer) {
+    if (!ok) {
+      for (int i = 0; i < n; ++i) {
+        hipError_t e = hipMemcpyAsync(p[i].host, p[i].dev, p[i].bytes, hipMemcpyDeviceToHost,
+                                      producer);
+        if (e != hipSuccess) return e;
+      }
+      return hipStreamSynchronize(producer);
+    }
+    const std::vector<Piece> c = chunks(p, n);
+    hipError_t err[kThreads] = {hipSuccess, hipSuccess};
+    if (hipEventRecord(done[0], producer) != hipSuccess) return hipErrorUnknown;
+    for (int t = 0; t < kThreads; ++t)
+      if (hipStreamWaitEvent(st[t], done[0], 0) != hipSuccess) return hipErrorUnknown;
+    auto work = [&](int t) {
+      hipError_t e = hipSetDevice(dev);
+      std::vector<size_t> mine;
+      for (size_t k = t; k < c.size(); k += kThreads) mine.push_back(k);
+      auto issue = [&](size_t j) {
+        const int s = (int)(j % kSlots);
+        hipError_t r = hipMemcpyAsync(pin[t][s], c[mine[j]].dev, c[mine[j]].bytes,
+                                      hipMemcpyDeviceToHost, st[t]);
+        if (r == hipSuccess) r = hipEventRecord(ev[t][s], st[t]);
+        return r;
+      };
+      if (e == hipSuccess && !mine.empty()) e = issue(0);
+      for (size_t j = 0; j < mine.size() && e == hipSuccess; ++j) {
+        if (j + 1 < mine.size()) e = issue(j + 1);  // the other slot: its copy-out is done
+        if (e == hipSuccess) e = hipEventSynchronize(ev[t][j % kSlots]);
+        if (e == hipSuccess) memcpy(c[mine[j]].host, pin[t][j % kSlots], c[mine[j]].bytes);
+      }
+      err[t] = e;
+    };
+    std::thread helper(work, 1);
+    work(0);
+    helper.join();
+    for (int t = 0; t < kThreads; ++t)
+      if (err[t] != hipSuccess) return err[t];
+    return hipSuccess;
+  }
+};
+
 // The chunk loop of one launch shape, captured as a graph and instantiated
 // once; later calls of the same shape replay it (updated in place when only
 // the arguments differ).
@@ -148,10 +284,11 @@ struct DevCtx {
   bool init = false;
   hipStream_t stream = nullptr;
   int n_cu = 0;
-  DevBuf ws, queue, flag;
+  DevBuf ws, queue, flag, descs;
   ChunkGraph graph;
   unsigned char* scratch = nullptr;  // this device's copy of `scratch`
   hipError_t scratch_err = hipSuccess;
+  Stager stage;                      // pinned staging of the host-array entry points
   // staging for the host-array entry points
   DevBuf C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -451,34 +588,6 @@ int record_solve(DevCtx& c, const GcBatchArgs& b, const GcRecord& rec_in, long* 
   return 0;
 }
 
-int adjoint_solve(DevCtx& c, const GcAdjArgs& a, hipStream_t st, double* tKernel) {
-  if (tKernel) *tKernel = 0.0;
-  if (a.nCell == 0) return 0;
-  const Tuning t = current_tuning();
-  int rc = check_perm(c, a.Perm, a.nCell, st);
-  if (rc) return rc;
-  hipError_t e;
-  const size_t slots = ((size_t)a.nCell + 63u) & ~(size_t)63u;
-  const GcVariantInfo vi = variant_info(t.variant);
-  GcAdjArgs aa = a;
-  // adjoint block + the segment buffers (every start states, step sizes)
-  aa.slot_doubles = vi.adj_doubles_per_slot + (long)a.every * (vi.nvar + 1);
-  if ((e = c.ws.ensure(slots * (size_t)aa.slot_doubles * sizeof(double))) != hipSuccess) {
-    set_error("hipMalloc(workspace)", e);
-    return -2;
-  }
-  if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
-  e = (t.variant == 2)   ? gc_launch_adjoint_v2(&aa, (double*)c.ws.p, 128, st)
-      : (t.variant == 3) ? gc_launch_adjoint_v3(&aa, (double*)c.ws.p, 128, st)
-                         : gc_launch_adjoint_v1(&aa, (double*)c.ws.p, 128, st);
-  if (e != hipSuccess) { set_error("launch", e); return -3; }
-  if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
-  if ((e = hipStreamSynchronize(st)) != hipSuccess) { set_error("adjoint", e); return -3; }
-  float ms = 0.f;
-  if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess && tKernel) *tKernel = ms * 1e-3;
-  return 0;
-}
-
 // Device context of the calling thread, initialised, locked by the caller;
 // orders the library stream after the caller's stream.
 DevCtx* enter(void* stream, int* rc) {
@@ -503,7 +612,250 @@ DevCtx* enter(void* stream, int* rc) {
   return cp;
 }
 
+// ---- device-resident batch across chemistry steps -----------------------------
+__global__ void k_iota1(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i + 1;
+}
+
+// sort keys of the next launch order: the cell's internal step count of the
+// step just solved (ISTAT(3)), and the warm start RCNTRL(3) = RSTAT(3) (Hnew)
+__global__ void k_carry_over(const int* ISTAT, const double* RSTAT, double* RCNTRL, int* keys,
+                             int n, int warm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = ISTAT[(size_t)i * 8 + 2];
+  if (warm) RCNTRL[(size_t)i * 20 + 2] = RSTAT[(size_t)i * 3 + 2];
+}
+
 }  // namespace
+
+// ---- several batches in one launch ------------------------------------------------
+namespace {
+
+int check_batches(int nBatch, const GcAmdBatchDesc* batch, unsigned* qn) {
+  if (nBatch < 1 || nBatch > 4096 || !batch) return -1;
+  unsigned long long tot = 0;
+  for (int k = 0; k < nBatch; ++k) {
+    if (batch[k].nCell < 0) return -1;
+    tot += (unsigned)batch[k].nCell;
+  }
+  if (tot > 0x7fffffffull) return -1;
+  *qn = (unsigned)tot;
+  return 0;
+}
+
+// Device descriptors of the batches (queue offsets = cells of the batches
+// before), every launch order checked; done (host-visible, nBatch counters,
+// may be null) counts each batch's finished cells.  Returns 0 or a negative
+// code.
+int upload_descs(DevCtx& c, int nBatch, const GcAmdBatchDesc* batch, hipStream_t st,
+                 unsigned* done) {
+  std::vector<GcBatchDesc> h((size_t)nBatch);
+  unsigned q0 = 0;
+  for (int k = 0; k < nBatch; ++k) {
+    const GcAmdBatchDesc& x = batch[k];
+    h[k].a = GcBatchArgs{x.nCell, x.Tstart, x.Tend, x.C, x.RCONST, x.ATOL, x.RTOL, x.ICNTRL,
+                         x.RCNTRL, x.Perm, x.ISTAT, x.RSTAT, x.IERR};
+    h[k].q0 = q0;
+    h[k].done = done ? done + k : nullptr;
+    q0 += (unsigned)x.nCell;
+    if (x.nCell > 0) {
+      const int rc = check_perm(c, x.Perm, x.nCell, st);
+      if (rc) return rc;
+    }
+  }
+  hipError_t e = c.descs.ensure(h.size() * sizeof(GcBatchDesc));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c.descs.p, h.data(), h.size() * sizeof(GcBatchDesc), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    set_error("batch descriptors", e);
+    return -2;
+  }
+  return 0;
+}
+
+// Finished-cell counters of a launch with progress reports: one per batch,
+// pinned host memory the kernel adds to with system-scope atomics and the
+// waiting thread reads.
+struct DoneCounters {
+  unsigned* p = nullptr;
+  ~DoneCounters() {
+    if (p) (void)hipHostFree(p);
+  }
+  int init(int n) {
+    if (hipHostMalloc((void**)&p, sizeof(unsigned) * (size_t)n, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      set_error("progress counters: hipHostMalloc");
+      return -2;
+    }
+    memset(p, 0, sizeof(unsigned) * (size_t)n);
+    return 0;
+  }
+};
+
+// Calls progress on each change of the finished-cell counts (polled every
+// 0.2 s) until `end` (recorded after the launch) has completed, and once more
+// at the end.  size(k) is batch k's cell count.
+template <class Size>
+void report_progress(hipEvent_t end, const unsigned* done, int nBatch, Size size, unsigned qn,
+                     gc_amd_progress_fn progress, void* user) {
+  long long last = -1;
+  for (;;) {
+    const hipError_t q = hipEventQuery(end);
+    int nb = 0;
+    long long cells = 0;
+    for (int k = 0; k < nBatch; ++k) {
+      const unsigned v = __atomic_load_n(done + k, __ATOMIC_RELAXED);
+      cells += v;
+      nb += v >= (unsigned)size(k);
+    }
+    if (cells != last) {
+      progress(nb, cells, (long long)qn, user);
+      last = cells;
+    }
+    if (q != hipErrorNotReady) break;
+    std::this_thread::sleep_for(std::chrono::milliseconds(200));
+  }
+}
+
+int multi_solve(DevCtx& c, int nBatch, const GcAmdBatchDesc* batch, unsigned qn,
+                const GcRecord* rec_in, long* used, long* yused, hipStream_t st,
+                double* tKernel, gc_amd_progress_fn progress = nullptr, void* user = nullptr) {
+  const Tuning t = current_tuning();
+  DoneCounters done;
+  if (progress && done.init(nBatch)) return -2;
+  int rc = upload_descs(c, nBatch, batch, st, done.p);
+  if (rc) return rc;
+  if (qn == 0) return 0;
+  hipError_t e;
+  const unsigned S = slots_for(c, GC_AMD_KERNEL_PERSISTENT, t, (int)qn);
+  if ((e = c.ws.ensure((size_t)S * (size_t)ws_per_slot() * sizeof(double))) != hipSuccess) {
+    set_error("hipMalloc(workspace)", e);
+    return -2;
+  }
+  if ((e = c.queue.ensure(256)) != hipSuccess) { set_error("hipMalloc(queue)", e); return -2; }
+  unsigned* q = (unsigned*)c.queue.p;
+  GcRecord rec{};
+  if (rec_in) {
+    rec = *rec_in;
+    rec.count = q + 1;
+    rec.ycount = q + 2;
+    hipLaunchKernelGGL(k_fill_i32, dim3((qn + 255) / 256), dim3(256), 0, st, rec.head, (int)qn, -1);
+  }
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, q, 3);
+  const unsigned waves = S / 64u;
+  unsigned fair = (qn + waves - 1u) / waves;
+  if (fair < 1u) fair = 1u;
+  const int grain = (unsigned)t.grain > fair ? (int)fair : t.grain;
+  const GcBatchDesc* d = (const GcBatchDesc*)c.descs.p;
+  double* ws = (double*)c.ws.p;
+  const GcRecord* rp = rec_in ? &rec : nullptr;
+  if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+  e = (t.variant == 2) ? gc_launch_persistent_multi_v2(d, nBatch, qn, ws, S, q, t.block, t.cells_per_wave, grain, rp, st)
+      : (t.variant == 3) ? gc_launch_persistent_multi_v3(d, nBatch, qn, ws, S, q, t.block, t.cells_per_wave, grain, rp, st)
+                         : gc_launch_persistent_multi_v1(d, nBatch, qn, ws, S, q, t.block, t.cells_per_wave, grain, rp, st);
+  if (e != hipSuccess) { set_error("launch", e); return -3; }
+  if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+  if (progress)
+    report_progress(c.ev1, done.p, nBatch, [&](int k) { return batch[k].nCell; }, qn, progress,
+                    user);
+  unsigned counts[2] = {0, 0};
+  if (rec_in &&
+      (e = hipMemcpyAsync(counts, rec.count, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st)) != hipSuccess) {
+    set_error("record counts", e);
+    return -3;
+  }
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) { set_error("solve", e); return -3; }
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess && tKernel) *tKernel = ms * 1e-3;
+  if (rec_in) {
+    if (used) *used = (long)counts[0];
+    if (yused) *yused = (long)counts[1];
+    if (counts[0] > rec.cap || counts[1] > rec.ycap) {
+      set_error("record pools too small: retry with cap >= *used and ycap >= *yused");
+      return -4;
+    }
+  }
+  return 0;
+}
+
+// The reverse sweep over the batches h (arguments filled in but for
+// slot_doubles; one record pool and `every` for all) in one persistent launch:
+// launch orders checked, one workspace slot per resident lane.
+int adjoint_sweep(DevCtx& c, std::vector<GcAdjDesc>& h, hipStream_t st,
+                  gc_amd_progress_fn progress, void* user, double* tKernel) {
+  if (tKernel) *tKernel = 0.0;
+  const int nb = (int)h.size();
+  const Tuning t = current_tuning();
+  const GcVariantInfo vi = variant_info(t.variant);
+  // adjoint block + the segment buffers (`every` start states and step sizes)
+  const long slot_doubles = vi.adj_doubles_per_slot + (long)h[0].a.every * (vi.nvar + 1);
+  DoneCounters done;
+  if (progress && done.init(nb)) return -2;
+  unsigned long long q0 = 0;
+  for (int k = 0; k < nb; ++k) {
+    h[k].a.slot_doubles = slot_doubles;
+    h[k].q0 = (unsigned)q0;
+    h[k].done = done.p ? done.p + k : nullptr;
+    q0 += (unsigned)h[k].a.nCell;
+    if (h[k].a.nCell > 0) {
+      const int rc = check_perm(c, h[k].a.Perm, h[k].a.nCell, st);
+      if (rc) return rc;
+    }
+  }
+  const unsigned qn = (unsigned)q0;
+  if (qn == 0) return 0;
+  // resident waves of the sweep kernel: its occupancy target (its LDS row
+  // segment fits that many per CU), or fewer when the launch tuning asks for
+  // fewer waves per CU; never more than there are 64-cell groups
+  const int block = 128;
+  const unsigned groups = (qn + 63u) / 64u;
+  int wpc = 4 * vi.waves_per_eu;
+  if (t.blocks_per_cu > 0 && t.blocks_per_cu * (t.block / 64) < wpc)
+    wpc = t.blocks_per_cu * (t.block / 64);
+  unsigned waves = (unsigned)(c.n_cu * wpc);
+  if (waves > groups) waves = groups;
+  waves = (waves + 1u) & ~1u;  // whole blocks of two waves
+  hipError_t e = c.descs.ensure(h.size() * sizeof(GcAdjDesc));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c.descs.p, h.data(), h.size() * sizeof(GcAdjDesc), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = c.ws.ensure((size_t)waves * 64u * (size_t)slot_doubles * sizeof(double));
+  if (e == hipSuccess) e = c.queue.ensure(256);
+  if (e != hipSuccess) {
+    set_error("adjoint sweep: device memory", e);
+    return -2;
+  }
+  unsigned* q = (unsigned*)c.queue.p;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, q, 1);
+  const GcAdjDesc* d = (const GcAdjDesc*)c.descs.p;
+  double* ws = (double*)c.ws.p;
+  if ((e = hipEventRecord(c.ev0, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+  e = (t.variant == 2)   ? gc_launch_adjoint_v2(d, nb, qn, ws, slot_doubles, waves, q, block, st)
+      : (t.variant == 3) ? gc_launch_adjoint_v3(d, nb, qn, ws, slot_doubles, waves, q, block, st)
+                         : gc_launch_adjoint_v1(d, nb, qn, ws, slot_doubles, waves, q, block, st);
+  if (e != hipSuccess) { set_error("launch", e); return -3; }
+  if ((e = hipEventRecord(c.ev1, st)) != hipSuccess) { set_error("hipEventRecord", e); return -3; }
+  if (progress)
+    report_progress(c.ev1, done.p, nb, [&](int k) { return h[k].a.nCell; }, qn, progress, user);
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) { set_error("adjoint", e); return -3; }
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess && tKernel) *tKernel = ms * 1e-3;
+  return 0;
+}
+
+}  // namespace
+
+struct GcAmdResident {
+  int dev = -1;
+  int n = 0;
+  int steps = 0;
+  DevBuf C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR;
+  DevBuf keys, keys_out, cells, sort_tmp;
+  size_t sort_bytes = 0;
+};
 
 extern "C" {
 
@@ -544,9 +896,11 @@ int gc_amd_adjoint_batch_device(int nCell, double Tstart, double Tend, const dou
   int rc = 0;
   DevCtx* cp = enter(stream, &rc);
   if (!cp) return rc;
-  GcAdjArgs a{nCell, Tstart, Tend, C, RCONST, ICNTRL, Perm, recY, recK, recH, recPrev, recYslot,
-              head, (unsigned)cap, (unsigned)ycap, every, 0, lamC, gC, gRCONST, IERR};
-  rc = adjoint_solve(*cp, a, cp->stream, tKernel);
+  std::vector<GcAdjDesc> h(1);
+  h[0].a = GcAdjArgs{nCell, Tstart, Tend, C, RCONST, ICNTRL, Perm, recY, recK, recH, recPrev,
+                     recYslot, head, (unsigned)cap, (unsigned)ycap, every, 0, lamC, gC, gRCONST,
+                     IERR};
+  rc = adjoint_sweep(*cp, h, cp->stream, nullptr, nullptr, tKernel);
   cp->mu.unlock();
   return rc;
 }
@@ -616,10 +970,14 @@ void gc_gpu_integrate_batch_dev(int dev, int nCell, double Tstart, double Tend, 
       {&c.C, C, n * nspec * 8}, {&c.RCONST, RCONST, n * md.nreact * 8},
       {&c.ATOL, ATOL, (size_t)md.nvar * 8}, {&c.RTOL, RTOL, (size_t)md.nvar * 8},
       {&c.ICNTRL, ICNTRL, n * 20 * 4}, {&c.RCNTRL, RCNTRL, n * 20 * 8}, {&c.Perm, Perm, n * 4}};
+  Piece pin[7];
+  int np = 0;
   for (auto& x : in) {
     if (e == hipSuccess) e = x.b->ensure(x.bytes);
-    if (e == hipSuccess) e = hipMemcpyAsync(x.b->p, x.h, x.bytes, hipMemcpyHostToDevice, c.stream);
+    pin[np++] = {x.b->p, const_cast<void*>(x.h), x.bytes};
   }
+  c.stage.init();
+  if (e == hipSuccess) e = c.stage.to_device(d, pin, np, c.stream, c.stream);
   if (e == hipSuccess) e = c.ISTAT.ensure(n * 8 * 4);
   if (e == hipSuccess) e = c.RSTAT.ensure(n * 3 * 8);
   if (e == hipSuccess) e = c.IERR.ensure(n * 4);
@@ -649,12 +1007,9 @@ void gc_gpu_integrate_batch_dev(int dev, int nCell, double Tstart, double Tend, 
     fill_ierr(IERR, nCell, -98);
     return;
   }
-  struct { void* h; const DevBuf* b; size_t bytes; } out[] = {
-      {C, &c.C, n * nspec * 8}, {ISTAT, &c.ISTAT, n * 8 * 4}, {RSTAT, &c.RSTAT, n * 3 * 8},
-      {IERR, &c.IERR, n * 4}};
-  for (auto& x : out)
-    if (e == hipSuccess) e = hipMemcpyAsync(x.h, x.b->p, x.bytes, hipMemcpyDeviceToHost, c.stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  Piece pout[4] = {{c.C.p, C, n * nspec * 8}, {c.ISTAT.p, ISTAT, n * 8 * 4},
+                   {c.RSTAT.p, RSTAT, n * 3 * 8}, {c.IERR.p, IERR, n * 4}};
+  if (e == hipSuccess) e = c.stage.to_host(d, pout, 4, c.stream);
   if (e != hipSuccess) {
     set_error("copy back", e);
     fill_ierr(IERR, nCell, -98);
@@ -784,6 +1139,255 @@ int gc_amd_integrate_batch_noalloc(int kernel, int nCell, double Tstart, double 
   if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess && tKernel) *tKernel = ms * 1e-3;
   c.mu.unlock();
   return 0;
+}
+
+int gc_amd_integrate_batch_masked(int kernel, int nCell, double Tstart, double Tend, double* C,
+                                  const double* RCONST, const double* ATOL, const double* RTOL,
+                                  const int* ICNTRL, const double* RCNTRL, const int* Perm,
+                                  const unsigned* doFun, const unsigned* doJvs,
+                                  const unsigned* doSlv, int* ISTAT, double* RSTAT, int* IERR,
+                                  void* stream, double* tKernel) {
+  if (tKernel) *tKernel = 0.0;
+  int rc = 0;
+  DevCtx* cp = enter(stream, &rc);
+  if (!cp) return rc;
+  GcBatchArgs b{nCell, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR};
+  b.doFun = doFun;
+  b.doJvs = doJvs;
+  b.doSlv = doSlv;
+  rc = device_solve(*cp, kernel, 0, b, cp->stream, tKernel);
+  cp->mu.unlock();
+  return rc;
+}
+
+int gc_amd_integrate_batches_device(int nBatch, const GcAmdBatchDesc* batch,
+                                    gc_amd_progress_fn progress, void* user, void* stream,
+                                    double* tKernel) {
+  if (tKernel) *tKernel = 0.0;
+  unsigned qn = 0;
+  if (check_batches(nBatch, batch, &qn) != 0) {
+    set_error("gc_amd_integrate_batches_device: bad argument");
+    return -1;
+  }
+  int rc = 0;
+  DevCtx* cp = enter(stream, &rc);
+  if (!cp) return rc;
+  rc = multi_solve(*cp, nBatch, batch, qn, nullptr, nullptr, nullptr, cp->stream, tKernel,
+                   progress, user);
+  cp->mu.unlock();
+  return rc;
+}
+
+int gc_amd_record_batches_device(int nBatch, const GcAmdBatchDesc* batch, double* recH,
+                                 int* recPrev, int* recYslot, int* head, long cap, double* recY,
+                                 long ycap, int every, double* recK, long* used, long* yused,
+                                 gc_amd_progress_fn progress, void* user, void* stream,
+                                 double* tKernel) {
+  if (tKernel) *tKernel = 0.0;
+  if (used) *used = 0;
+  if (yused) *yused = 0;
+  unsigned qn = 0;
+  if (check_batches(nBatch, batch, &qn) != 0 || cap < 0 || cap > 0x7fffffffL || ycap < 0 ||
+      ycap > 0x7fffffffL || every < 1 || every > 4096) {
+    set_error("gc_amd_record_batches_device: bad argument");
+    return -1;
+  }
+  int rc = 0;
+  DevCtx* cp = enter(stream, &rc);
+  if (!cp) return rc;
+  GcRecord rec{recH, recPrev, recYslot, head, recY, recK, nullptr, nullptr,
+               (unsigned)cap, (unsigned)ycap, every};
+  rc = multi_solve(*cp, nBatch, batch, qn, &rec, used, yused, cp->stream, tKernel, progress,
+                   user);
+  cp->mu.unlock();
+  return rc;
+}
+
+int gc_amd_adjoint_batches_device(int nBatch, const GcAmdAdjBatch* batch, const double* recH,
+                                  const int* recPrev, const int* recYslot, const int* head,
+                                  long cap, const double* recY, long ycap, int every,
+                                  const double* recK, gc_amd_progress_fn progress, void* user,
+                                  void* stream, double* tKernel) {
+  if (tKernel) *tKernel = 0.0;
+  unsigned long long tot = 0;
+  bool bad = nBatch < 1 || nBatch > 4096 || !batch || cap < 0 || cap > 0x7fffffffL || ycap < 0 ||
+             ycap > 0x7fffffffL || every < 1 || every > 64;
+  for (int k = 0; !bad && k < nBatch; ++k) {
+    bad = batch[k].nCell < 0;
+    tot += bad ? 0u : (unsigned)batch[k].nCell;
+  }
+  if (bad || tot > 0x7fffffffull) {
+    set_error("gc_amd_adjoint_batches_device: bad argument");
+    return -1;
+  }
+  int rc = 0;
+  DevCtx* cp = enter(stream, &rc);
+  if (!cp) return rc;
+  std::vector<GcAdjDesc> h((size_t)nBatch);
+  for (int k = 0; k < nBatch; ++k) {
+    const GcAmdAdjBatch& x = batch[k];
+    h[k].a = GcAdjArgs{x.nCell, x.Tstart, x.Tend, x.C, x.RCONST, x.ICNTRL, x.Perm, recY, recK,
+                       recH, recPrev, recYslot, head, (unsigned)cap, (unsigned)ycap, every, 0,
+                       x.lamC, x.gC, x.gRCONST, x.IERR};
+  }
+  rc = adjoint_sweep(*cp, h, cp->stream, progress, user, tKernel);
+  cp->mu.unlock();
+  return rc;
+}
+
+int gc_amd_resident_create(int nCell, const int* ICNTRL, const double* RCNTRL,
+                           const double* ATOL, const double* RTOL, GcAmdResident** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  if (nCell < 1 || !ICNTRL || !RCNTRL || !ATOL || !RTOL) {
+    set_error("gc_amd_resident_create: bad argument");
+    return -1;
+  }
+  int rc = 0;
+  DevCtx* cp = enter(nullptr, &rc);
+  if (!cp) return rc;
+  DevCtx& c = *cp;
+  GcAmdResident* r = new (std::nothrow) GcAmdResident;
+  if (!r) {
+    c.mu.unlock();
+    set_error("gc_amd_resident_create: host memory");
+    return -2;
+  }
+  int d = -1;
+  (void)hipGetDevice(&d);
+  r->dev = d;
+  r->n = nCell;
+  const MechDims md = mech_dims();
+  const size_t n = (size_t)nCell, nspec = (size_t)(md.nvar + md.nfix);
+  hipError_t e = hipSuccess;
+  DevBuf* bufs[] = {&r->C, &r->RCONST, &r->ATOL, &r->RTOL, &r->ICNTRL, &r->RCNTRL, &r->Perm,
+                    &r->ISTAT, &r->RSTAT, &r->IERR, &r->keys, &r->keys_out, &r->cells};
+  const size_t bytes[] = {n * nspec * 8, n * md.nreact * 8, (size_t)md.nvar * 8,
+                          (size_t)md.nvar * 8, n * 20 * 4, n * 20 * 8, n * 4, n * 8 * 4,
+                          n * 3 * 8, n * 4, n * 4, n * 4, n * 4};
+  for (int i = 0; i < 13 && e == hipSuccess; ++i) e = bufs[i]->ensure(bytes[i]);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, r->sort_bytes, (int*)r->keys.p,
+                                                     (int*)r->keys_out.p, (int*)r->cells.p,
+                                                     (int*)r->Perm.p, nCell, 0, 32, c.stream);
+  if (e == hipSuccess) e = r->sort_tmp.ensure(r->sort_bytes > 0 ? r->sort_bytes : 16);
+  Piece pin[4] = {{r->ICNTRL.p, const_cast<int*>(ICNTRL), n * 20 * 4},
+                  {r->RCNTRL.p, const_cast<double*>(RCNTRL), n * 20 * 8},
+                  {r->ATOL.p, const_cast<double*>(ATOL), (size_t)md.nvar * 8},
+                  {r->RTOL.p, const_cast<double*>(RTOL), (size_t)md.nvar * 8}};
+  c.stage.init();
+  if (e == hipSuccess) e = c.stage.to_device(d, pin, 4, c.stream, c.stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_iota1, dim3((nCell + 255) / 256), dim3(256), 0, c.stream, (int*)r->Perm.p,
+                       nCell);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  c.mu.unlock();
+  if (e != hipSuccess) {
+    set_error("gc_amd_resident_create", e);
+    for (DevBuf* b : bufs) b->release();
+    r->sort_tmp.release();
+    delete r;
+    return -2;
+  }
+  *out = r;
+  return 0;
+}
+
+int gc_amd_resident_step(GcAmdResident* r, double Tstart, double Tend, const double* C,
+                         const double* RCONST, int warm, double* Cout, int* ISTAT,
+                         double* RSTAT, int* IERR, int* PermUsed, double* tTotal,
+                         double* tKernel) {
+  if (tTotal) *tTotal = 0.0;
+  if (tKernel) *tKernel = 0.0;
+  if (!r || r->n < 1) {
+    set_error("gc_amd_resident_step: bad handle");
+    return -1;
+  }
+  if (r->steps == 0 && !C) {
+    set_error("gc_amd_resident_step: the first step needs C");
+    return -1;
+  }
+  if (r->steps == 0 && !RCONST) {
+    set_error("gc_amd_resident_step: the first step needs RCONST");
+    return -1;
+  }
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != r->dev) {
+    set_error("gc_amd_resident_step: the batch lives on another device");
+    return -1;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = 0;
+  DevCtx* cp = enter(nullptr, &rc);
+  if (!cp) return rc;
+  DevCtx& c = *cp;
+  const MechDims md = mech_dims();
+  const int nCell = r->n;
+  const size_t n = (size_t)nCell, nspec = (size_t)(md.nvar + md.nfix);
+  hipError_t e = hipSuccess;
+  // the launch order and warm start from the step before, on the device
+  if (r->steps > 0) {
+    hipLaunchKernelGGL(k_carry_over, dim3((nCell + 255) / 256), dim3(256), 0, c.stream,
+                       (const int*)r->ISTAT.p, (const double*)r->RSTAT.p, (double*)r->RCNTRL.p,
+                       (int*)r->keys.p, nCell, warm);
+    hipLaunchKernelGGL(k_iota1, dim3((nCell + 255) / 256), dim3(256), 0, c.stream,
+                       (int*)r->cells.p, nCell);
+    e = hipGetLastError();
+    size_t tb = r->sort_bytes;
+    if (e == hipSuccess)
+      e = hipcub::DeviceRadixSort::SortPairsDescending(r->sort_tmp.p, tb, (int*)r->keys.p,
+                                                       (int*)r->keys_out.p, (int*)r->cells.p,
+                                                       (int*)r->Perm.p, nCell, 0, 32, c.stream);
+  }
+  Piece pin[2];
+  int np = 0;
+  if (C) pin[np++] = {r->C.p, const_cast<double*>(C), n * nspec * 8};
+  if (RCONST) pin[np++] = {r->RCONST.p, const_cast<double*>(RCONST), n * md.nreact * 8};
+  if (e == hipSuccess && np) e = c.stage.to_device(r->dev, pin, np, c.stream, c.stream);
+  if (e != hipSuccess) {
+    set_error("gc_amd_resident_step: staging", e);
+    c.mu.unlock();
+    return -3;
+  }
+  GcBatchArgs b{nCell, Tstart, Tend, (double*)r->C.p, (const double*)r->RCONST.p,
+                (const double*)r->ATOL.p, (const double*)r->RTOL.p, (const int*)r->ICNTRL.p,
+                (const double*)r->RCNTRL.p, (const int*)r->Perm.p, (int*)r->ISTAT.p,
+                (double*)r->RSTAT.p, (int*)r->IERR.p};
+  rc = device_solve(c, GC_AMD_KERNEL_PERSISTENT, 0, b, c.stream, tKernel);
+  if (rc != 0) {
+    c.mu.unlock();
+    return rc;
+  }
+  r->steps += 1;
+  Piece pout[5];
+  np = 0;
+  if (Cout) pout[np++] = {r->C.p, Cout, n * nspec * 8};
+  if (ISTAT) pout[np++] = {r->ISTAT.p, ISTAT, n * 8 * 4};
+  if (RSTAT) pout[np++] = {r->RSTAT.p, RSTAT, n * 3 * 8};
+  if (IERR) pout[np++] = {r->IERR.p, IERR, n * 4};
+  if (PermUsed) pout[np++] = {r->Perm.p, PermUsed, n * 4};
+  e = np ? c.stage.to_host(r->dev, pout, np, c.stream) : hipStreamSynchronize(c.stream);
+  c.mu.unlock();
+  if (e != hipSuccess) {
+    set_error("gc_amd_resident_step: copy back", e);
+    return -3;
+  }
+  if (tTotal) *tTotal = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+void gc_amd_resident_destroy(GcAmdResident* r) {
+  if (!r) return;
+  int cur = -1;
+  if (hipGetDevice(&cur) == hipSuccess && cur != r->dev) (void)hipSetDevice(r->dev);
+  DevBuf* bufs[] = {&r->C, &r->RCONST, &r->ATOL, &r->RTOL, &r->ICNTRL, &r->RCNTRL, &r->Perm,
+                    &r->ISTAT, &r->RSTAT, &r->IERR, &r->keys, &r->keys_out, &r->cells,
+                    &r->sort_tmp};
+  for (DevBuf* b : bufs) b->release();
+  if (cur >= 0 && cur != r->dev) (void)hipSetDevice(cur);
+  delete r;
 }
 
 int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave, int grain, int variant) {

@@ -111,6 +111,13 @@ struct Blk {
   __device__ __forceinline__ void st(int soff, double v) const {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
   }
+  // the low 32 bits of an element (the mask words)
+  __device__ __forceinline__ unsigned ld32(int soff) const {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+  }
+  __device__ __forceinline__ void st32(int soff, unsigned v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
+  }
   __device__ __forceinline__ double lds_ld(int off) const {
     return *(double*)((char*)lds + off);
   }
@@ -130,6 +137,23 @@ __device__ __forceinline__ Blk make_blk(double* ws, unsigned wave, int lane, dou
   b.lds = lds_wave + lane;
   return b;
 }
+
+// One auto-reduce flag of the lane's cell (masked kernels): flag k of the
+// mask whose first element is `elem0` is bit k%32 of element elem0 + k/32.
+// Statements are visited in ascending (or descending) flag order, so the last
+// word read is kept; k is wave-uniform (a table word), so is the reload test.
+struct MaskWord {
+  int w = -1;
+  unsigned bits = 0u;
+  __device__ __forceinline__ bool on(const Blk& b, int elem0, int k) {
+    const int wi = k >> 5;
+    if (wi != w) {
+      w = wi;
+      bits = b.ld32((elem0 + wi) * 512);
+    }
+    return ((bits >> (k & 31)) & 1u) != 0u;
+  }
+};
 
 // Element passes: val(k) loads (and combines) what element k needs, put(k, v)
 // stores it.  VB consecutive elements are evaluated, their loads issued
@@ -287,10 +311,16 @@ __device__ __forceinline__ double sum_dterms(const Blk& b, ctab_t*& t, int n) {
 // Fun_SPLIT (gckpp_Function.F90:2170-5060): rates, then per species P_VAR,
 // D_VAR (term products evaluated in place) and Vdot = P_VAR - D_VAR*V,
 // stored at out + i*512.
+// Masked (M): IF (DO_FUN(i)) guards both sums of species i
+// (gckpp_Function.F90:3280-5056); a species whose flag is off keeps the P_VAR
+// and D_VAR the cell's workspace holds, zero from the cell's start, so its
+// Vdot is 0 - 0*V.
+template <bool M>
 __device__ void fun(const Blk& b, bool ynew, int out) {
   if (ynew) run_products(b, gc_prod_rate_yn, gc_prodgrp_rate_yn);
   else run_products(b, gc_prod_rate_y, gc_prodgrp_rate_y);
   ctab_t* t = tab(ynew ? gc_pd_yn : gc_pd_y);
+  MaskWord mf;
   for (int i = 0; i < NVAR; ++i) {
     const i4 h = ld4(t), f = ld4(t + 4);  // header and the first term
     const int nP = h.x, nD = h.y, vo = h.z, oo = h.w;
@@ -301,7 +331,11 @@ __device__ void fun(const Blk& b, bool ynew, int out) {
     } else {
       t -= 4;
     }
-    const double D = nD > 0 ? sum_dterms(b, t, nD) : 0.0;
+    double D = nD > 0 ? sum_dterms(b, t, nD) : 0.0;
+    if (M && !mf.on(b, MECH_E_MFUN, oo >> 9)) {
+      P = 0.0;
+      D = 0.0;
+    }
     b.st(out + oo, P - D * b.ld(vo));
   }
 }
@@ -309,28 +343,36 @@ __device__ void fun(const Blk& b, bool ynew, int out) {
 // One term {soff, flags, coef} of an ENTRYG record with its operand x: the
 // entry's sum restarts at a first term; at its last term the prepared entry
 // -sum (+ ghinv on the diagonal) goes to its LDS slot.
+// Masked (M): IF (DO_JVS(k)) guards the entry's sum (gckpp_Jacobian.F90:
+// 3676-20887); an entry whose flag is off keeps the Jac0 value of the cell's
+// workspace, zero from the cell's start, so Ghimj(k) = -0.0 (+ ghinv on the
+// diagonal).
+template <bool M>
 __device__ __forceinline__ void entry_term(const Blk& b, i4 t, double x, double ghinv,
-                                           double& acc, int& sing) {
+                                           double& acc, int& sing, MaskWord& mj) {
   const int fl = t.y;
   const double cx = dbl(t.z, t.w) * x;
   acc = (fl & 1) ? cx : acc + cx;
   if (fl & 2) {
+    if (M && !mj.on(b, MECH_E_MJVS, fl >> 14)) acc = 0.0;
     double g = -acc;
     if (fl & 4) {
       g = g + ghinv;
       sing |= (__builtin_fabs(g) < TINYD) ? 1 : 0;
     }
-    b.lds_st((fl >> 8) * 512, g);
+    b.lds_st(((fl >> 8) & 63) * 512, g);
   }
 }
 
 // Jac_SP fused with Ghimj = -Jac0, diag += ghinv and KppDecomp, row segments
 // in LDS (the lu table format is described in codegen/gen_tables.py).
 // Returns 1 when a prepared diagonal entry is below TINY.
+template <bool M>
 __device__ int jac_lu(const Blk& b, double ghinv) {
   run_products(b, gc_prod_bjac, gc_prodgrp_bjac);
   ctab_t* op = tab(GC_LU_TABLE);
   int sing = 0;
+  MaskWord mj;
   for (;;) {
     const i4 h = ld4(op), f = ld4(op + 4);  // header and the first 4 words after it
     const int code = h.x;
@@ -350,7 +392,7 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) x[r] = b.ld(tt[r].x);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) entry_term(b, tt[r], x[r], ghinv, acc, sing);
+        for (int r = 0; r < 8; ++r) entry_term<M>(b, tt[r], x[r], ghinv, acc, sing, mj);
       }
       for (; q < T; q += 4, op += 16) {
         i4 tt[4];
@@ -360,7 +402,7 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) x[r] = b.ld(tt[r].x);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) entry_term(b, tt[r], x[r], ghinv, acc, sing);
+        for (int r = 0; r < 4; ++r) entry_term<M>(b, tt[r], x[r], ghinv, acc, sing, mj);
       }
     } else if (code == 2 || code == 3) {
       // n updates {slot, U} (n a multiple of 4; padding entries update the
@@ -468,13 +510,17 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
 
 // Rows {n, x_i, diag, 0} + n/2 {LU, x_j, LU, x_j} (n a multiple of 4):
 // x_i = x_i - LU*x_j ... in order, then / diag for the backward rows.
-template <bool DIV>
-__device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows) {
+// Masked (M): IF (DO_SLV(i)) guards the row (gckpp_LinearAlgebra.F90:
+// 653-2309), x_i stays as it was; the rows' x_i are elements xbase + i.
+template <bool DIV, bool M = false>
+__device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows, int xbase = 0) {
+  MaskWord ms;
   for (int r = 0; r < rows; ++r) {
     const i4 h = ld4(t), p01 = ld4(t + 4), p23 = ld4(t + 8);  // header, first four pairs
     const int n = h.x, xi = h.y, dg = h.z;
     t += 4;
     double x = b.ld(xi);
+    const double x0 = x;
     if (n > 0) {
       const double l0 = b.ld(p01.x), x0 = b.ld(p01.y), l1 = b.ld(p01.z), x1 = b.ld(p01.w);
       const double l2 = b.ld(p23.x), x2 = b.ld(p23.y), l3 = b.ld(p23.z), x3 = b.ld(p23.w);
@@ -510,6 +556,7 @@ __device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows) {
       x = x - l3 * x3;
     }
     if (DIV) x = x / b.ld(dg);
+    if (M && !ms.on(b, MECH_E_MSLV, (xi >> 9) - xbase)) x = x0;
     b.st(xi, x);
   }
 }
@@ -521,13 +568,16 @@ __device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows) {
 // column order), / U(i,i); a row of that suffix also leaves x_i in its slot
 // (lds_i >= 0), before any later row reads it.  LDS padding pairs read the
 // spare slot, set to 0.0 here: x - 0.0*0.0 == x.
-__device__ __forceinline__ void solve_bwd_lds(const Blk& b, ctab_t* t, int rows) {
+template <bool M>
+__device__ __forceinline__ void solve_bwd_lds(const Blk& b, ctab_t* t, int rows, int xbase) {
   b.lds_st(GC_SEG * 512, 0.0);
+  MaskWord ms;
   for (int r = 0; r < rows; ++r) {
     const i4 h = ld4(t);
     t += 4;
     const int nc = h.x & 0xffff, nh = h.x >> 16;
     double x = b.ld(h.y);
+    const double x0 = x;
     int u = 0;
     for (; u + 8 <= nc; u += 8, t += 16) {
       i4 q[4];
@@ -580,20 +630,23 @@ __device__ __forceinline__ void solve_bwd_lds(const Blk& b, ctab_t* t, int rows)
       x = x - l3 * x3;
     }
     x = x / b.ld(h.z);
+    if (M && !ms.on(b, MECH_E_MSLV, (h.y >> 9) - xbase)) x = x0;
     b.st(h.y, x);
     if (h.w >= 0) b.lds_st(h.w, x);
   }
 }
 
 // KppSolve (gckpp_LinearAlgebra.F90:645-2311) in place on stage vector K_s.
+template <bool M>
 __device__ void solve(const Blk& b, int s) {
   const int* fw = s == 0 ? gc_fwd0 : s == 1 ? gc_fwd1 : s == 2 ? gc_fwd2 : gc_fwd3;
   const int* bw = s == 0   ? GC_CAT(gc_bwd0_, GC_SEG)
                   : s == 1 ? GC_CAT(gc_bwd1_, GC_SEG)
                   : s == 2 ? GC_CAT(gc_bwd2_, GC_SEG)
                            : GC_CAT(gc_bwd3_, GC_SEG);
-  solve_rows<false>(b, tab(fw), MECH_NFWD);
-  solve_bwd_lds(b, tab(bw), MECH_NBWD);
+  const int xbase = MECH_E_K + s * NVAR;
+  solve_rows<false, M>(b, tab(fw), MECH_NFWD, xbase);
+  solve_bwd_lds<M>(b, tab(bw), MECH_NBWD, xbase);
 }
 
 // Options after the merge of gc_gpu_batch_api.F90:79-88 and the decode of
@@ -631,8 +684,21 @@ __device__ int decode_opts(const int* icu, const double* rcu, double Tstart, dou
   return 0;
 }
 
+// The cell's auto-reduce flags into its mask elements (a null mask: every
+// flag on): DO_FUN, DO_JVS, DO_SLV words of the caller's per-cell arrays.
+__device__ void gather_masks(const GcBatchArgs& a, long cell, const Blk& b) {
+  for (int w = 0; w < MECH_MW_FUN; ++w)
+    b.st32((MECH_E_MFUN + w) * 512, a.doFun ? a.doFun[cell * MECH_MW_FUN + w] : ~0u);
+  for (int w = 0; w < MECH_MW_JVS; ++w)
+    b.st32((MECH_E_MJVS + w) * 512, a.doJvs ? a.doJvs[cell * MECH_MW_JVS + w] : ~0u);
+  for (int w = 0; w < MECH_MW_SLV; ++w)
+    b.st32((MECH_E_MSLV + w) * 512, a.doSlv ? a.doSlv[cell * MECH_MW_SLV + w] : ~0u);
+}
+
 // Copy cell `cell` (0-based column of the caller arrays) into the lane.
+template <bool M>
 __device__ void gather(const GcBatchArgs& a, long cell, const Blk& b) {
+  if (M) gather_masks(a, cell, b);
   const double* c = a.C + cell * NSPEC;
   const double* r = a.RCONST + cell * NREACT;
   vpass<NVAR>([&](int i) { return c[i]; }, [&](int i, double v) { b.st(O_Y + i * 512, v); });
@@ -653,12 +719,13 @@ __device__ void scatter(const GcBatchArgs& a, long cell, const Blk& b, const int
 
 // Fcn0 <- Fun(Y); for non-autonomous problems also the dFdT replica
 // (gckpp_BatchIntegrator.F90:549-561).
+template <bool M>
 __device__ void step_functions(const Blk& b, const Opts& o, double T, int* ist) {
-  fun(b, false, O_FCN0);
+  fun<M>(b, false, O_FCN0);
   ist[0] += 1;
   if (!o.autonomous) {
     const double Delta = sqrt(ROUNDOFF) * dmax(1.0e-6, fabs(T));
-    fun(b, false, O_DFDT);
+    fun<M>(b, false, O_DFDT);
     ist[0] += 1;
     vpass<NVAR>(
         [&](int k) { return (b.ld(O_DFDT + k * 512) - b.ld(O_FCN0 + k * 512)) / Delta; },
@@ -679,6 +746,7 @@ __device__ void stage_point(const Blk& b, int s) {
 
 // The four stages after a successful factorisation
 // (gckpp_BatchIntegrator.F90:594-643).
+template <bool M>
 __device__ void stages(const Blk& b, bool autonomous, double Dir, double H, int* ist) {
   for (int s = 0; s < 4; ++s) {
     const int Ks = O_K + s * O_VEC;
@@ -690,7 +758,7 @@ __device__ void stages(const Blk& b, bool autonomous, double Dir, double H, int*
       src = O_FCN0;
     } else {
       stage_point(b, s);
-      fun(b, true, Ks);
+      fun<M>(b, true, Ks);
       ist[0] += 1;
     }
     double HC[3];
@@ -707,16 +775,17 @@ __device__ void stages(const Blk& b, bool autonomous, double Dir, double H, int*
       vpass<NVAR>([&](int k) { return b.ld(Ks + k * 512) + HG * b.ld(O_DFDT + k * 512); },
                   [&](int k, double v) { b.st(Ks + k * 512, v); });
     }
-    solve(b, s);
+    solve<M>(b, s);
     ist[6] += 1;
   }
 }
 
 // The stages, then Ynew, Yerr and the error norm
 // (gckpp_BatchIntegrator.F90:594-665).  Returns Err.
+template <bool M>
 __device__ double stages_and_error(const Blk& b, const Opts& o, double Dir, double H,
                                    const double* atol, const double* rtol, int* ist) {
-  stages(b, o.autonomous, Dir, H, ist);
+  stages<M>(b, o.autonomous, Dir, H, ist);
   double Err = 0.0;
   struct YE {
     double y, yn, ye;
@@ -757,6 +826,7 @@ __device__ __forceinline__ bool time_left(int dir, double T, double Tend) {
 }
 
 // plain: Integrate_Cell_S per thread, with its own nested loops.
+template <bool M>
 __device__ int integrate_cell(const Blk& b, const Opts& o, double Tstart, double Tend,
                               const double* atol, const double* rtol, int* ist, double* rst) {
   int Nsteps = 0;
@@ -771,13 +841,13 @@ __device__ int integrate_cell(const Blk& b, const Opts& o, double Tstart, double
     if (Nsteps > o.max_steps) return -6;
     if (((T + 0.1 * H) == T) || (H <= ROUNDOFF)) return -7;
     H = dmin(H, fabs(Tend - T));
-    step_functions(b, o, T, ist);
+    step_functions<M>(b, o, T, ist);
     ist[1] += 1;  // Jac_SP of this step (evaluated in each attempt below)
     for (;;) {    // UntilAccepted
       int ncons = 0;
       for (;;) {  // singular retries
         const double ghinv = 1.0 / ((Dir * H) * ros_Gamma[0]);
-        const int sing = jac_lu(b, ghinv);
+        const int sing = jac_lu<M>(b, ghinv);
         ist[5] += 1;
         if (!sing) break;
         ist[7] += 1;
@@ -785,7 +855,7 @@ __device__ int integrate_cell(const Blk& b, const Opts& o, double Tstart, double
         if (ncons <= 5) H = H * 0.5;
         else return -8;
       }
-      const double Err = stages_and_error(b, o, Dir, H, atol, rtol, ist);
+      const double Err = stages_and_error<M>(b, o, Dir, H, atol, rtol, ist);
       const double Fac = dmin(o.FacMax, dmax(o.FacMin, o.FacSafe / gc_pow_third(Err)));
       double Hnew = H * Fac;
       Nsteps += 1;
@@ -817,11 +887,8 @@ __device__ int integrate_cell(const Blk& b, const Opts& o, double Tstart, double
 
 enum : int { PH_FREE = 0, PH_STEP = 1, PH_RETRY = 2, PH_SING = 3 };
 
-}  // namespace
-
-extern "C" __global__ void
-__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
-GC_SFX(gc_rodas3_plain)(GcBatchArgs a, double* ws) {
+template <bool M>
+__device__ __forceinline__ void plain_body(const GcBatchArgs& a, double* ws) {
   extern __shared__ double lds[];
   const unsigned ic = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -830,11 +897,26 @@ GC_SFX(gc_rodas3_plain)(GcBatchArgs a, double* ws) {
   const long cell = (long)a.Perm[ic] - 1;
   int ist[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double rst[3] = {0.0, 0.0, 0.0};
-  gather(a, cell, b);
+  gather<M>(a, cell, b);
   Opts o;
   int ierr = decode_opts(a.ICNTRL + cell * 20, a.RCNTRL + cell * 20, a.Tstart, a.Tend, o);
-  if (ierr == 0) ierr = integrate_cell(b, o, a.Tstart, a.Tend, a.ATOL, a.RTOL, ist, rst);
+  if (ierr == 0) ierr = integrate_cell<M>(b, o, a.Tstart, a.Tend, a.ATOL, a.RTOL, ist, rst);
   scatter(a, cell, b, ist, rst, ierr);
+}
+
+}  // namespace
+
+extern "C" __global__ void
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_plain)(GcBatchArgs a, double* ws) {
+  plain_body<false>(a, ws);
+}
+
+// The plain kernel with the cells' auto-reduce masks (DO_FUN, DO_JVS, DO_SLV).
+extern "C" __global__ void
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_plain_masked)(GcBatchArgs a, double* ws) {
+  plain_body<true>(a, ws);
 }
 
 namespace {
@@ -877,10 +959,17 @@ __device__ __forceinline__ void record_step(const Blk& b, const GcRecord& rec, b
   head = (int)slot;
 }
 
-template <bool REC>
+// The persistent grid over one batch `a`, or (MB) over nb batches described
+// by descs in one queue of qn positions: batch k's cells take positions
+// descs[k].q0 .. descs[k].q0 + descs[k].a.nCell - 1 in its own launch order,
+// each lane working with its current cell's batch arguments.  A cell's record
+// chain head (recording form) is head[q0 + cell].
+template <bool REC, bool M = false, bool MB = false>
 __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws, unsigned S,
                                                 unsigned* queue, int cells_per_wave, int grain,
-                                                const GcRecord& rec) {
+                                                const GcRecord& rec,
+                                                const GcBatchDesc* descs = nullptr, int nb = 0,
+                                                unsigned qn = 0) {
   extern __shared__ double lds[];
   const unsigned slot = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -898,6 +987,16 @@ __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws
   bool exhausted = !lane_on;
   unsigned rsv = 0, rsv_end = 0;  // wave-uniform reservation [rsv, rsv_end)
   int head = -1;                  // last record of the lane's cell (recording form)
+  GcBatchArgs la = a;             // the arguments of the lane's cell's batch
+  unsigned q0 = 0;                // its first queue position
+  unsigned* ldone = nullptr;      // its finished-cell counter (MB, may be null)
+  int bi = 0;                     // its index (MB)
+  const unsigned qtot = MB ? qn : (unsigned)a.nCell;
+  // a cell's results are out: count it for the caller's progress report
+  auto finished = [&]() {
+    if (MB && ldone)
+      __hip_atomic_fetch_add(ldone, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  };
 
   for (;;) {
     // ---- refill free lanes from the queue (cost order) -------------------
@@ -920,22 +1019,31 @@ __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws
       const unsigned rank = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
       if (need && rank < avail) {
         const unsigned q = rsv + rank;
-        if (q < (unsigned)a.nCell) {
-          cell = (long)a.Perm[q] - 1;
+        if (q < qtot) {
+          if (MB) {  // queue positions only grow, so does the batch index
+            while (bi + 1 < nb && q >= descs[bi + 1].q0) ++bi;
+            la = descs[bi].a;
+            q0 = descs[bi].q0;
+            ldone = descs[bi].done;
+          }
+          cell = (long)la.Perm[q - q0] - 1;
           for (int k = 0; k < 8; ++k) ist[k] = 0;
           for (int k = 0; k < 3; ++k) rst[k] = 0.0;
-          gather(a, cell, b);
-          const int e = decode_opts(a.ICNTRL + cell * 20, a.RCNTRL + cell * 20, a.Tstart, a.Tend, o);
+          gather<M>(la, cell, b);
+          const int e = decode_opts(la.ICNTRL + cell * 20, la.RCNTRL + cell * 20, la.Tstart,
+                                    la.Tend, o);
           if (e != 0) {
-            scatter(a, cell, b, ist, rst, e);
+            scatter(la, cell, b, ist, rst, e);
+            if (REC) rec.head[q0 + cell] = -1;
+            finished();
             cell = -1;  // stays free: takes another cell
           } else {
             head = -1;
             Nsteps = 0;
-            T = a.Tstart;
+            T = la.Tstart;
             H = dmin(dmax(fabs(o.Hmin), fabs(o.Hstart)), fabs(o.Hmax));
             if (fabs(H) <= 10.0 * ROUNDOFF) H = DELTAMIN;
-            dir = (a.Tend >= a.Tstart) ? +1 : -1;
+            dir = (la.Tend >= la.Tstart) ? +1 : -1;
             Dir = (double)dir;
             H = Dir * H;
             rejLast = rejMore = false;
@@ -946,27 +1054,28 @@ __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws
         }
       }
       rsv += (cnt < avail) ? cnt : avail;
-      if (rsv >= (unsigned)a.nCell && rsv >= rsv_end && phase == PH_FREE) exhausted = true;
+      if (rsv >= qtot && rsv >= rsv_end && phase == PH_FREE) exhausted = true;
     }
     if (!__any(phase != PH_FREE)) break;
 
     // ---- top of TimeLoop for lanes starting a step -----------------------
     if (phase == PH_STEP) {
       int ierr_now = 0;
-      if (!time_left(dir, T, a.Tend)) ierr_now = 1;
+      if (!time_left(dir, T, la.Tend)) ierr_now = 1;
       else if (Nsteps > o.max_steps) ierr_now = -6;
       else if (((T + 0.1 * H) == T) || (H <= ROUNDOFF)) ierr_now = -7;
       if (ierr_now != 0) {
-        scatter(a, cell, b, ist, rst, ierr_now);
-        if (REC) rec.head[cell] = head;
+        scatter(la, cell, b, ist, rst, ierr_now);
+        if (REC) rec.head[q0 + cell] = head;
+        finished();
         phase = PH_FREE;
         cell = -1;
       } else {
-        H = dmin(H, fabs(a.Tend - T));
+        H = dmin(H, fabs(la.Tend - T));
       }
     }
     if (phase == PH_STEP) {
-      step_functions(b, o, T, ist);
+      step_functions<M>(b, o, T, ist);
       ist[1] += 1;
     }
     if (phase == PH_STEP || phase == PH_RETRY) ncons = 0;
@@ -978,7 +1087,7 @@ __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws
     bool factored = false;
     if (phase != PH_FREE) {
       const double ghinv = 1.0 / ((Dir * H) * ros_Gamma[0]);
-      const int sing = jac_lu(b, ghinv);
+      const int sing = jac_lu<M>(b, ghinv);
       ist[5] += 1;
       if (sing) {
         ist[7] += 1;
@@ -987,8 +1096,9 @@ __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws
           H = H * 0.5;
           phase = PH_SING;
         } else {
-          scatter(a, cell, b, ist, rst, -8);
-          if (REC) rec.head[cell] = head;
+          scatter(la, cell, b, ist, rst, -8);
+          if (REC) rec.head[q0 + cell] = head;
+          finished();
           phase = PH_FREE;
           cell = -1;
         }
@@ -999,7 +1109,7 @@ __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws
     bool accepted = false;
     double Hnew = 0.0;
     if (factored) {
-      const double Err = stages_and_error(b, o, Dir, H, a.ATOL, a.RTOL, ist);
+      const double Err = stages_and_error<M>(b, o, Dir, H, la.ATOL, la.RTOL, ist);
       const double Fac = dmin(o.FacMax, dmax(o.FacMin, o.FacSafe / gc_pow_third(Err)));
       Hnew = H * Fac;
       Nsteps += 1;
@@ -1040,6 +1150,34 @@ __attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE,
 GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* queue,
                              int cells_per_wave, int grain) {
   persistent_body<false>(a, ws, S, queue, cells_per_wave, grain, GcRecord{});
+}
+
+// One persistent launch over nb batches (their own arrays, sizes, launch
+// orders and time spans) in one queue of qn positions.
+extern "C" __global__ void
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_persistent_multi)(const GcBatchDesc* descs, int nb, unsigned qn, double* ws,
+                                   unsigned S, unsigned* queue, int cells_per_wave, int grain) {
+  persistent_body<false, false, true>(GcBatchArgs{}, ws, S, queue, cells_per_wave, grain,
+                                      GcRecord{}, descs, nb, qn);
+}
+
+// Its recording form (heads of the record chains at head[q0 + cell]).
+extern "C" __global__ void
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_record_multi)(const GcBatchDesc* descs, int nb, unsigned qn, double* ws,
+                               unsigned S, unsigned* queue, int cells_per_wave, int grain,
+                               GcRecord rec) {
+  persistent_body<true, false, true>(GcBatchArgs{}, ws, S, queue, cells_per_wave, grain, rec,
+                                     descs, nb, qn);
+}
+
+// The persistent kernel with the cells' auto-reduce masks.
+extern "C" __global__ void
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_persistent_masked)(GcBatchArgs a, double* ws, unsigned S, unsigned* queue,
+                                    int cells_per_wave, int grain) {
+  persistent_body<false, true>(a, ws, S, queue, cells_per_wave, grain, GcRecord{});
 }
 
 // Recording form: also keeps every accepted step's start state and H for the
@@ -1311,21 +1449,19 @@ __device__ void adj_step(const Blk& b, double Dir, double H, bool clamp) {
 
 }  // namespace
 
-// One thread per cell in Perm order: walks the cell's records from the last
+// One lane per cell in Perm order: walks the cell's records from the last
 // to the first one segment at a time (the steps back to a checkpoint),
 // replays the segment forward from its checkpoint to regain each step's start
 // state, then recomputes each step's factors and stages and sweeps the adjoint
 // through it, last step first.
-extern "C" __global__ void
-__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
-GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
-  extern __shared__ double lds[];
-  const unsigned ic = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const Blk b = make_blk(ws, ic >> 6, lane, lds + (threadIdx.x >> 6) * ((GC_SEG + 1) * 64),
-                         (int)a.slot_doubles);
-  if (ic >= (unsigned)a.nCell) return;
-  const long cell = (long)a.Perm[ic] - 1;
+namespace {
+
+// The reverse sweep of one cell: launch position ic of batch `a`, whose cells
+// start at position q0 (cell Perm[ic - q0] - 1 of the batch, its record chain
+// head at head[q0 + cell]); b is the lane's workspace slot.
+__device__ __forceinline__ void adjoint_cell(const GcAdjArgs& a, unsigned q0, unsigned ic,
+                                             const Blk& b) {
+  const long cell = (long)a.Perm[ic - q0] - 1;
   const double* c = a.C + cell * NSPEC;
   const double* rc = a.RCONST + cell * NREACT;
   const double* lam = a.lamC + cell * NSPEC;
@@ -1357,7 +1493,7 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
   const int O_SEG = MECH_E_ADJ_TOTAL * 512;
   const int O_SEGH = O_SEG + m * NVAR * 512;
   unsigned walked = 0;
-  int slot = a.head[cell];
+  int slot = a.head[q0 + cell];
   while (slot >= 0 && ierr == 1) {
     // one segment, walked backwards: records up to and including a checkpoint
     int len = 0, ys = -1;
@@ -1391,12 +1527,12 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
                   [&](int i, double v) { b.st(O_SEG + (q * NVAR + i) * 512, v); });
       if (q == len - 1) break;
       const double H = b.ld(O_SEGH + (len - 1 - q) * 512);
-      fun(b, false, O_FCN0);
-      if (jac_lu(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
+      fun<false>(b, false, O_FCN0);
+      if (jac_lu<false>(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
         ierr = -8;
         break;
       }
-      stages(b, true, Dir, H, ist);
+      stages<false>(b, true, Dir, H, ist);
       vpass<NVAR>(  // Ynew and the accepted update, as the stepper
           [&](int k) {
             double yn = b.ld(O_Y + k * 512);
@@ -1418,13 +1554,13 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
         vpass<4 * NVAR>([&](int k) { return kk[k]; },
                         [&](int k, double v) { b.st(O_K + k * 512, v); });
       } else {
-        fun(b, false, O_FCN0);
+        fun<false>(b, false, O_FCN0);
       }
-      if (jac_lu(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
+      if (jac_lu<false>(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
         ierr = -8;
         break;
       }
-      if (!kept) stages(b, true, Dir, H, ist);
+      if (!kept) stages<false>(b, true, Dir, H, ist);
       adj_step(b, Dir, H, clamp);
     }
   }
@@ -1432,6 +1568,40 @@ GC_SFX(gc_rodas3_adjoint)(GcAdjArgs a, double* ws) {
   for (int j = 0; j < NFIX; ++j) gc[NVAR + j] = b.ld(O_GF + j * 512);
   vpass<NREACT>([&](int r) { return b.ld(O_GK + r * 512); }, [&](int r, double v) { gk[r] = v; });
   a.IERR[cell] = ierr;
+}
+
+}  // namespace
+
+// The sweep over nb batches (GcAdjDesc, qn cells in all) as a persistent
+// grid: each wave owns one workspace slot block and takes 64 consecutive
+// launch positions at a time from the queue counter (zeroed by the caller)
+// until the positions run out, so the workspace is bounded by the resident
+// waves, not by the cells.  descs[k].done (host-visible, or null) counts the
+// finished cells of batch k.
+extern "C" __global__ void
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_adjoint)(const GcAdjDesc* descs, int nb, unsigned qn, double* ws,
+                          long slot_doubles, unsigned* queue) {
+  extern __shared__ double lds[];
+  const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const Blk b = make_blk(ws, wave, lane, lds + (threadIdx.x >> 6) * ((GC_SEG + 1) * 64),
+                         (int)slot_doubles);
+  const unsigned groups = (qn + 63u) >> 6;
+  for (;;) {
+    unsigned g = 0;
+    if (lane == 0) g = atomicAdd(queue, 1u);
+    g = (unsigned)__builtin_amdgcn_readfirstlane((int)g);
+    if (g >= groups) break;
+    const unsigned ic = g * 64u + (unsigned)lane;
+    if (ic < qn) {
+      int bi = 0;
+      while (bi + 1 < nb && ic >= descs[bi + 1].q0) ++bi;
+      adjoint_cell(descs[bi].a, descs[bi].q0, ic, b);
+      if (descs[bi].done)
+        __hip_atomic_fetch_add(descs[bi].done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1442,7 +1612,10 @@ extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_plain)(const GcBatchArgs* a, 
   if (a->nCell <= 0) return hipSuccess;
   const unsigned grid = ((unsigned)a->nCell + (unsigned)block - 1u) / (unsigned)block;
   const size_t lds = (size_t)(block / 64) * (GC_SEG + 1) * 64 * sizeof(double);
-  hipLaunchKernelGGL(GC_SFX(gc_rodas3_plain), dim3(grid), dim3(block), lds, st, *a, ws);
+  if (a->doFun || a->doJvs || a->doSlv)
+    hipLaunchKernelGGL(GC_SFX(gc_rodas3_plain_masked), dim3(grid), dim3(block), lds, st, *a, ws);
+  else
+    hipLaunchKernelGGL(GC_SFX(gc_rodas3_plain), dim3(grid), dim3(block), lds, st, *a, ws);
   return hipGetLastError();
 }
 
@@ -1453,8 +1626,12 @@ extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_persistent)(const GcBatchArgs
   if (a->nCell <= 0) return hipSuccess;
   const unsigned grid = S / (unsigned)block;
   const size_t lds = (size_t)(block / 64) * (GC_SEG + 1) * 64 * sizeof(double);
-  hipLaunchKernelGGL(GC_SFX(gc_rodas3_persistent), dim3(grid), dim3(block), lds, st, *a, ws, S,
-                     queue, cells_per_wave, grain);
+  if (a->doFun || a->doJvs || a->doSlv)
+    hipLaunchKernelGGL(GC_SFX(gc_rodas3_persistent_masked), dim3(grid), dim3(block), lds, st, *a,
+                       ws, S, queue, cells_per_wave, grain);
+  else
+    hipLaunchKernelGGL(GC_SFX(gc_rodas3_persistent), dim3(grid), dim3(block), lds, st, *a, ws, S,
+                       queue, cells_per_wave, grain);
   return hipGetLastError();
 }
 
@@ -1470,12 +1647,32 @@ extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_record)(const GcBatchArgs* a,
   return hipGetLastError();
 }
 
-extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_adjoint)(const GcAdjArgs* a, double* ws,
-                                                          int block, hipStream_t st) {
-  if (a->nCell <= 0) return hipSuccess;
-  const unsigned grid = ((unsigned)a->nCell + (unsigned)block - 1u) / (unsigned)block;
+extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_persistent_multi)(
+    const GcBatchDesc* descs, int nb, unsigned qn, double* ws, unsigned S, unsigned* queue,
+    int block, int cells_per_wave, int grain, const GcRecord* rec, hipStream_t st) {
+  if (qn == 0) return hipSuccess;
+  const unsigned grid = S / (unsigned)block;
   const size_t lds = (size_t)(block / 64) * (GC_SEG + 1) * 64 * sizeof(double);
-  hipLaunchKernelGGL(GC_SFX(gc_rodas3_adjoint), dim3(grid), dim3(block), lds, st, *a, ws);
+  if (rec)
+    hipLaunchKernelGGL(GC_SFX(gc_rodas3_record_multi), dim3(grid), dim3(block), lds, st, descs,
+                       nb, qn, ws, S, queue, cells_per_wave, grain, *rec);
+  else
+    hipLaunchKernelGGL(GC_SFX(gc_rodas3_persistent_multi), dim3(grid), dim3(block), lds, st,
+                       descs, nb, qn, ws, S, queue, cells_per_wave, grain);
+  return hipGetLastError();
+}
+
+extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_adjoint)(const GcAdjDesc* descs, int nb,
+                                                          unsigned qn, double* ws,
+                                                          long slot_doubles, unsigned waves,
+                                                          unsigned* queue, int block,
+                                                          hipStream_t st) {
+  if (qn == 0) return hipSuccess;
+  const unsigned wpb = (unsigned)block / 64u;
+  const unsigned grid = (waves + wpb - 1u) / wpb;
+  const size_t lds = (size_t)wpb * (GC_SEG + 1) * 64 * sizeof(double);
+  hipLaunchKernelGGL(GC_SFX(gc_rodas3_adjoint), dim3(grid), dim3(block), lds, st, descs, nb, qn,
+                     ws, slot_doubles, queue);
   return hipGetLastError();
 }
 

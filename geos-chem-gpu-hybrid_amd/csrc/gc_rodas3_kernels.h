@@ -19,6 +19,20 @@ struct GcBatchArgs {
   int* ISTAT;
   double* RSTAT;
   int* IERR;
+  // auto-reduce masks, per cell (NULL: every flag on): DO_FUN(NVAR),
+  // DO_JVS(LU_NONZERO), DO_SLV(NVAR) as 32-bit words, flag k in bit k%32 of
+  // word k/32, ceil(n/32) words per cell; any non-null selects the masked kernels
+  const unsigned* doFun = nullptr;
+  const unsigned* doJvs = nullptr;
+  const unsigned* doSlv = nullptr;
+};
+
+// One batch of a multi-batch launch: its arguments and its first position in
+// the launch's queue (the number of cells of the batches before it).
+struct GcBatchDesc {
+  GcBatchArgs a;
+  unsigned q0;
+  unsigned* done;  // cells of the batch finished so far (host-visible), or null
 };
 
 // What a compiled variant is: mechanism dimensions, doubles of workspace per
@@ -67,12 +81,21 @@ struct GcAdjArgs {
   unsigned cap;          // records in the pool (every slot followed is checked)
   unsigned ycap;         // states in the state pool
   int every;             // checkpoint interval of the recording
-  long slot_doubles;     // workspace per cell (forward + adjoint + segment)
+  long slot_doubles;     // workspace per lane (forward + adjoint + segment)
   const double* lamC;    // (NSPEC, nCell): dL/dC at Tend
   double* gC;            // (NSPEC, nCell): dL/dC at Tstart
   double* gRCONST;       // (NREACT, nCell): dL/dRCONST
   int* IERR;             // (nCell): 1, -8 singular step, -3 non-autonomous,
                          // -5 a record chain that does not fit the pools
+};
+
+// One batch of an adjoint sweep: its arguments, its first launch position
+// (the number of cells of the batches before it) and its finished-cell
+// counter (host-visible, or null).
+struct GcAdjDesc {
+  GcAdjArgs a;
+  unsigned q0;
+  unsigned* done;
 };
 
 // Internal to the library (not exported): the C-ABI in include/gcchemgpu.h is
@@ -88,9 +111,13 @@ struct GcAdjArgs {
   extern "C" GC_INTERNAL hipError_t gc_launch_record_v##V(                            \
       const GcBatchArgs* a, double* ws, unsigned S, unsigned* queue, int block,       \
       int cells_per_wave, int grain, const GcRecord* rec, hipStream_t st);            \
-  extern "C" GC_INTERNAL hipError_t gc_launch_adjoint_v##V(const GcAdjArgs* a,          \
-                                                         double* ws, int block,       \
-                                                         hipStream_t st);             \
+  extern "C" GC_INTERNAL hipError_t gc_launch_persistent_multi_v##V(                  \
+      const GcBatchDesc* descs, int nb, unsigned qn, double* ws, unsigned S,           \
+      unsigned* queue, int block, int cells_per_wave, int grain, const GcRecord* rec,  \
+      hipStream_t st);                                                                 \
+  extern "C" GC_INTERNAL hipError_t gc_launch_adjoint_v##V(                            \
+      const GcAdjDesc* descs, int nb, unsigned qn, double* ws, long slot_doubles,      \
+      unsigned waves, unsigned* queue, int block, hipStream_t st);                     \
   extern "C" GC_INTERNAL void gc_variant_info_v##V(GcVariantInfo* d);
 
 GC_DECLARE_VARIANT(1)

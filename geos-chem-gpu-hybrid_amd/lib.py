@@ -40,6 +40,22 @@ class GcAmdHybrid(ctypes.Structure):
         return st
 
 
+class GcAmdBatchDesc(ctypes.Structure):
+    """struct GcAmdBatchDesc of include/gcchemgpu.h (one batch of a multi-batch launch)."""
+    _fields_ = [("nCell", I), ("Tstart", D), ("Tend", D), ("C", P), ("RCONST", P), ("ATOL", P),
+                ("RTOL", P), ("ICNTRL", P), ("RCNTRL", P), ("Perm", P), ("ISTAT", P),
+                ("RSTAT", P), ("IERR", P)]
+
+
+class GcAmdAdjBatch(ctypes.Structure):
+    """struct GcAmdAdjBatch of include/gcchemgpu.h (one batch of a multi-batch sweep)."""
+    _fields_ = [("nCell", I), ("Tstart", D), ("Tend", D), ("C", P), ("RCONST", P),
+                ("ICNTRL", P), ("Perm", P), ("lamC", P), ("gC", P), ("gRCONST", P), ("IERR", P)]
+
+
+# gc_amd_progress_fn: void (*)(int batchesDone, long long cellsDone, long long cellsTotal, void*)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, I, ctypes.c_longlong, ctypes.c_longlong, P)
+
 # gc_amd_cpu_tail_fn: void (*)(int first, int last, void *user)
 CPU_TAIL_FN = ctypes.CFUNCTYPE(None, I, I, P)
 
@@ -52,6 +68,16 @@ SIGNATURES = {
     "gc_gpu_selftest": (None, [IP, IP, DP]),
     "gc_amd_integrate_batch_device": (I, [I, I, I, D, D, P, P, P, P, P, P, P, P, P, P, P, DP]),
     "gc_amd_integrate_batch_noalloc": (I, [I, I, D, D, P, P, P, P, P, P, P, P, P, P, P, DP]),
+    "gc_amd_integrate_batch_masked": (I, [I, I, D, D] + [P] * 14 + [DP]),
+    "gc_amd_integrate_batches_device": (I, [I, P, P, P, P, DP]),
+    "gc_amd_record_batches_device": (I, [I, P, P, P, P, P, ctypes.c_long, P, ctypes.c_long, I, P,
+                                         ctypes.POINTER(ctypes.c_long),
+                                         ctypes.POINTER(ctypes.c_long), P, P, P, DP]),
+    "gc_amd_adjoint_batches_device": (I, [I, P, P, P, P, P, ctypes.c_long, P, ctypes.c_long, I,
+                                          P, P, P, P, DP]),
+    "gc_amd_resident_create": (I, [I, P, P, P, P, ctypes.POINTER(ctypes.c_void_p)]),
+    "gc_amd_resident_step": (I, [P, D, D, P, P, I, P, P, P, P, P, DP, DP]),
+    "gc_amd_resident_destroy": (None, [P]),
     "gc_amd_record_batch_device": (I, [I, D, D] + [P] * 14 + [ctypes.c_long, P, ctypes.c_long, I,
                                        P, ctypes.POINTER(ctypes.c_long),
                                        ctypes.POINTER(ctypes.c_long), P, DP]),

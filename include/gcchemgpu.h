@@ -133,9 +133,153 @@ int gc_amd_integrate_batch_noalloc(int kernel, int nCell, double Tstart,
                                    const int *Perm, int *ISTAT, double *RSTAT,
                                    int *IERR, void *stream, double *tKernel);
 
+/* Batch solve with the cells' auto-reduce flags, arrays in device memory as
+ * for gc_amd_integrate_batch_device (chunk = 0).  The reference's generated
+ * mechanism code guards each species' production and loss sums with
+ * DO_FUN(i) (gckpp_Function.F90:29, 3280-5056), each Jacobian entry with
+ * DO_JVS(k) (gckpp_Jacobian.F90:31, 3676-20887) and each KppSolve row with
+ * DO_SLV(i) (gckpp_LinearAlgebra.F90:31, 653-2309), all .TRUE. in the batch
+ * build; here they are per-cell data.  A flag that is off skips its statement
+ * exactly as the guard does: P_VAR(i)/D_VAR(i) and JVS(k) keep the value the
+ * cell's workspace holds, which is zero from the cell's start (so a masked
+ * species has Vdot = 0 - 0*V and a masked entry Ghimj = -0.0, + ghinv on the
+ * diagonal), and a masked row leaves X(i) as it is.  Masks are 32-bit words,
+ * flag k (0-based) in bit k%32 of word k/32, per cell (cell-major, the
+ * caller's column index): doFun ceil(NVAR/32) words, doJvs
+ * ceil(LU_NONZERO/32), doSlv ceil(NVAR/32).  A null mask means every flag
+ * on.  The flags are the caller's choice: choosing them (the stock
+ * integrator's auto-reduce threshold rule) stays with the caller, and
+ * ICNTRL(12) = 1 still returns IERR -90.  No counterpart in the reference's
+ * batch library, whose path has no auto-reduce.  Returns as
+ * gc_amd_integrate_batch_device. */
+int gc_amd_integrate_batch_masked(int kernel, int nCell, double Tstart,
+                                  double Tend, double *C, const double *RCONST,
+                                  const double *ATOL, const double *RTOL,
+                                  const int *ICNTRL, const double *RCNTRL,
+                                  const int *Perm, const unsigned *doFun,
+                                  const unsigned *doJvs, const unsigned *doSlv,
+                                  int *ISTAT, double *RSTAT, int *IERR,
+                                  void *stream, double *tKernel);
+
+/* ---- several batches in one launch ---------------------------------------
+ * Independent batches -- each with its own arrays, size, launch order and
+ * time span, each with the meaning of one gc_amd_integrate_batch_device call
+ * (the per-call semantics of gpulib/src/gc_gpu_batch_api.F90:147-216) --
+ * solved by one persistent launch whose work queue runs through batch 0's
+ * launch order, then batch 1's, and so on, so lanes freed in one batch take
+ * cells of the next.  No counterpart in the reference (one batch per call). */
+typedef struct GcAmdBatchDesc {
+  int nCell;
+  double Tstart, Tend;
+  double *C;                      /* (NSPEC, nCell) inout, device memory */
+  const double *RCONST;           /* (NREACT, nCell) */
+  const double *ATOL, *RTOL;      /* (NVAR) */
+  const int *ICNTRL;              /* (20, nCell) */
+  const double *RCNTRL;           /* (20, nCell) */
+  const int *Perm;                /* (nCell), 1-based launch order */
+  int *ISTAT;                     /* (8, nCell) out */
+  double *RSTAT;                  /* (3, nCell) out */
+  int *IERR;                      /* (nCell) out */
+} GcAmdBatchDesc;
+
+/* Progress of a multi-batch launch: batches whose cells have all finished,
+ * cells finished, cells in all batches. */
+typedef void (*gc_amd_progress_fn)(int batchesDone, long long cellsDone,
+                                   long long cellsTotal, void *user);
+
+/* Solve nBatch (1..4096) batches in one persistent launch.  progress (may be
+ * NULL) is called on the calling thread whenever the number of finished
+ * cells has changed, polled every 0.2 s while the launch runs, and once at
+ * its end.  Returns 0 or the codes of gc_amd_integrate_batch_device (-1 also
+ * for a bad descriptor). */
+int gc_amd_integrate_batches_device(int nBatch, const GcAmdBatchDesc *batch,
+                                    gc_amd_progress_fn progress, void *user,
+                                    void *stream, double *tKernel);
+
+/* The recording form of gc_amd_integrate_batches_device for the discrete
+ * adjoint: one record pool for all batches (arguments as in
+ * gc_amd_record_batch_device, progress as above); head has one entry per
+ * cell of all batches, batch k's cell c (0-based column) at
+ * head[n_0 + ... + n_{k-1} + c]. */
+int gc_amd_record_batches_device(int nBatch, const GcAmdBatchDesc *batch,
+                                 double *recH, int *recPrev, int *recYslot,
+                                 int *head, long cap, double *recY, long ycap,
+                                 int every, double *recK, long *used, long *yused,
+                                 gc_amd_progress_fn progress, void *user,
+                                 void *stream, double *tKernel);
+
+/* One batch of a multi-batch adjoint sweep (as gc_amd_adjoint_batch_device). */
+typedef struct GcAmdAdjBatch {
+  int nCell;
+  double Tstart, Tend;
+  const double *C;
+  const double *RCONST;
+  const int *ICNTRL;
+  const int *Perm;
+  const double *lamC;             /* (NSPEC, nCell) dL/dC at Tend */
+  double *gC;                     /* (NSPEC, nCell) dL/dC at Tstart */
+  double *gRCONST;                /* (NREACT, nCell) dL/dRCONST */
+  int *IERR;                      /* (nCell) */
+} GcAmdAdjBatch;
+
+/* Reverse sweep over the batches of one gc_amd_record_batches_device call in
+ * one launch (same pools, head and `every`, 1..64); progress (may be NULL) as
+ * in gc_amd_integrate_batches_device.  Returns as
+ * gc_amd_adjoint_batch_device. */
+int gc_amd_adjoint_batches_device(int nBatch, const GcAmdAdjBatch *batch,
+                                  const double *recH, const int *recPrev,
+                                  const int *recYslot, const int *head, long cap,
+                                  const double *recY, long ycap, int every,
+                                  const double *recK, gc_amd_progress_fn progress,
+                                  void *user, void *stream, double *tKernel);
+
+/* ---- device-resident batch across chemistry steps ------------------------
+ * The reference stages the whole batch host->device on every call and keeps
+ * nothing on the device (gpulib/src/gc_gpu_batch_api.F90:199-211); its caller
+ * carries each cell's step size Hnew (RSTAT(3) -> State_Chm%KPPHvalue -> next
+ * step's RCNTRL(3), src/geos-chem/GeosCore/fullchem_mod.F90:1775, 2044) and
+ * internal step count (ISTAT(3) -> BatchPrevNsteps, the next step's cost
+ * sort key, :2522-2535 and :905-981) across steps on the host.  A resident
+ * batch keeps all of it in HBM instead: the batch arrays, the controls, the
+ * warm-start step sizes and the cost keys; the next step's launch order (the
+ * stable descending sort by the previous step's ISTAT(3)) is computed on the
+ * device, so the caller keeps its cells in its own (grid) order.  Host
+ * arrays move through pinned buffers, host copies overlapping the transfers.
+ * No counterpart in the reference. */
+typedef struct GcAmdResident GcAmdResident;
+
+/* A resident batch of nCell cells on the calling thread's current device,
+ * with the controls ICNTRL(20,nCell), RCNTRL(20,nCell), ATOL(NVAR) and
+ * RTOL(NVAR) uploaded once (layouts of gc_gpu_integrate_batch).  Returns 0,
+ * -1 bad argument, -2 device memory, -3 runtime error. */
+int gc_amd_resident_create(int nCell, const int *ICNTRL, const double *RCNTRL,
+                           const double *ATOL, const double *RTOL,
+                           GcAmdResident **out);
+
+/* One chemistry step of the resident batch over [Tstart, Tend] with the
+ * persistent kernel.  C(NSPEC,nCell) and RCONST(NREACT,nCell) are host arrays
+ * uploaded for this step, or NULL to keep what the device holds (C: the
+ * previous step's result; both are required on the first step).  From the
+ * second step on, the launch order is the stable descending sort of the
+ * cells by the previous step's ISTAT(3) (identity on the first step), and
+ * with warm != 0 each cell starts from the previous step's Hnew:
+ * RCNTRL(3) = RSTAT(3).  Cout, ISTAT, RSTAT, IERR (host, NULL to skip) receive
+ * the results as gc_gpu_integrate_batch returns them, PermUsed (NULL to skip)
+ * the 1-based launch order the step used.  tTotal / tKernel: seconds with and
+ * without the host copies.  Returns 0 or the codes of
+ * gc_amd_integrate_batch_device.  Blocking. */
+int gc_amd_resident_step(GcAmdResident *r, double Tstart, double Tend,
+                         const double *C, const double *RCONST, int warm,
+                         double *Cout, int *ISTAT, double *RSTAT, int *IERR,
+                         int *PermUsed, double *tTotal, double *tKernel);
+
+/* Free a resident batch's device memory. */
+void gc_amd_resident_destroy(GcAmdResident *r);
+
 /* Launch tuning of the persistent kernel (values <= 0 keep the current one):
  *   block           work-group size (64..256, multiple of 64)
- *   blocks_per_cu   resident work-groups per compute unit (grid = CUs * this)
+ *   blocks_per_cu   resident work-groups per compute unit (grid = CUs * this;
+ *                   the adjoint sweep runs at most block/64 * this waves per CU)
  *   cells_per_wave  lanes of each 64-wide wave that take cells (16..64)
  *   grain           queue positions a wave claims per atomic (1..4096)
  *   variant         LU row-segment variant: 1 = 32-entry segments in LDS
@@ -187,8 +331,10 @@ int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double *C,
  * ICNTRL(1) must be non-zero (autonomous, as GEOS-Chem sets it) and
  * ICNTRL(16) = 1 (clamp) is honoured.  IERR(c) = 1, -3 for a non-autonomous
  * cell (gradients 0), -8 if a recorded step's matrix is singular, -5 if the
- * cell's chain does not fit the pools.  Returns 0 or a negative code as
- * above.  Blocking; ordered after `stream`. */
+ * cell's chain does not fit the pools.  The sweep runs as a persistent grid
+ * (64 cells in Perm order at a time per wave), so its workspace is one slot
+ * per resident lane whatever nCell.  Returns 0 or a negative code as above.
+ * Blocking; ordered after `stream`. */
 int gc_amd_adjoint_batch_device(int nCell, double Tstart, double Tend,
                                 const double *C, const double *RCONST,
                                 const int *ICNTRL, const int *Perm,

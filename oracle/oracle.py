@@ -136,24 +136,54 @@ class Oracle:
         L.or_solve_cell_rec.argtypes = [P, P, P, P, D, D, P, P, P, P, P, P, P, P, P, I, P]
         L.or_integrate_batch.restype = D
         L.or_integrate_batch.argtypes = [P, I, D, D, P, P, P, P, P, P, P, P, P, P, I]
+        L.or_fun_split_m.argtypes = [P] * 9
+        L.or_jac_sp_m.argtypes = [P] * 7
+        L.or_solve_m.argtypes = [P] * 4
+        L.or_integrate_batch_m.restype = D
+        L.or_integrate_batch_m.argtypes = [P, I, D, D] + [P] * 10 + [I, P, P, P]
 
     @property
     def _m(self):
         return ctypes.byref(self.mech)
 
     # -- kernels ------------------------------------------------------------
-    def fun_split(self, V, F, RCT):
+    # Masks (auto-reduce flags): uint32 words, flag k in bit k%32 of word k//32
+    # (see mask_words); None = every flag on.  A guarded output whose flag is
+    # off keeps the value passed in (P/D for Fun, JVS for Jac; zeros if None).
+    @staticmethod
+    def mask_words(flags) -> np.ndarray:
+        """Pack booleans flags[..., n] into uint32 words[..., ceil(n/32)]."""
+        f = np.asarray(flags, bool)
+        n = f.shape[-1]
+        pad = np.zeros(f.shape[:-1] + ((-n) % 32,), bool)
+        bits = np.concatenate([f, pad], axis=-1).reshape(f.shape[:-1] + (-1, 32))
+        return (bits.astype(np.uint64) << np.arange(32, dtype=np.uint64)).sum(-1).astype(np.uint32)
+
+    def fun_split(self, V, F, RCT, do_fun=None, P0=None, D0=None):
         V, F, RCT = _f64(V), _f64(F), _f64(RCT)
-        vdot, p, d = (np.zeros(self.nvar) for _ in range(3))
+        vdot = np.zeros(self.nvar)
+        p = np.zeros(self.nvar) if P0 is None else _f64(P0).copy()
+        d = np.zeros(self.nvar) if D0 is None else _f64(D0).copy()
         a = np.zeros(self.nreact)
-        self.lib.or_fun_split(self._m, _ptr(V), _ptr(F), _ptr(RCT), _ptr(vdot),
-                              _ptr(p), _ptr(d), _ptr(a))
+        if do_fun is None:
+            self.lib.or_fun_split(self._m, _ptr(V), _ptr(F), _ptr(RCT), _ptr(vdot),
+                                  _ptr(p), _ptr(d), _ptr(a))
+        else:
+            mk = np.ascontiguousarray(do_fun, np.uint32)
+            self.lib.or_fun_split_m(self._m, _ptr(V), _ptr(F), _ptr(RCT), _ptr(vdot),
+                                    _ptr(p), _ptr(d), _ptr(a), _ptr(mk))
         return vdot, p, d, a
 
-    def jac_sp(self, V, F, RCT):
+    def jac_sp(self, V, F, RCT, do_jvs=None, JVS0=None):
         V, F, RCT = _f64(V), _f64(F), _f64(RCT)
-        jvs, b = np.zeros(self.nnz), np.zeros(max(self.nb, 1))
-        self.lib.or_jac_sp(self._m, _ptr(V), _ptr(F), _ptr(RCT), _ptr(jvs), _ptr(b))
+        jvs = np.zeros(self.nnz) if JVS0 is None else _f64(JVS0).copy()
+        b = np.zeros(max(self.nb, 1))
+        if do_jvs is None:
+            self.lib.or_jac_sp(self._m, _ptr(V), _ptr(F), _ptr(RCT), _ptr(jvs), _ptr(b))
+        else:
+            mk = np.ascontiguousarray(do_jvs, np.uint32)
+            self.lib.or_jac_sp_m(self._m, _ptr(V), _ptr(F), _ptr(RCT), _ptr(jvs), _ptr(b),
+                                 _ptr(mk))
         return jvs
 
     def decomp(self, jvs):
@@ -163,10 +193,14 @@ class Oracle:
         self.lib.or_decomp(self._m, _ptr(lu), _ptr(w), ctypes.byref(ier))
         return lu, ier.value
 
-    def solve(self, lu, x):
+    def solve(self, lu, x, do_slv=None):
         lu = _f64(lu)
         x = _f64(x).copy()
-        self.lib.or_solve(self._m, _ptr(lu), _ptr(x))
+        if do_slv is None:
+            self.lib.or_solve(self._m, _ptr(lu), _ptr(x))
+        else:
+            mk = np.ascontiguousarray(do_slv, np.uint32)
+            self.lib.or_solve_m(self._m, _ptr(lu), _ptr(x), _ptr(mk))
         return x
 
     def solve_tr(self, lu, x):
@@ -207,10 +241,13 @@ class Oracle:
         return Y
 
     def integrate_batch(self, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL,
-                        Perm=None, nthreads=0):
+                        Perm=None, nthreads=0, masks=None):
         """gc_gpu_integrate_batch semantics.  Arrays are cell-major:
         C[nCell, NSPEC] (updated in place on a copy, returned), RCONST[nCell,
         NREACT], ICNTRL/RCNTRL[nCell, 20], Perm 1-based (identity if None).
+        masks: None or (doFun[nCell, ceil(NVAR/32)], doJvs[nCell,
+        ceil(LU_NONZERO/32)], doSlv[nCell, ceil(NVAR/32)]) uint32 words, any
+        of them None for every flag on.
         Returns (C, ISTAT[nCell,8], RSTAT[nCell,3], IERR[nCell], seconds)."""
         C = _f64(C).copy()
         n = C.shape[0]
@@ -222,8 +259,24 @@ class Oracle:
         ISTAT = np.zeros((n, 8), np.int32)
         RSTAT = np.zeros((n, 3))
         IERR = np.zeros(n, np.int32)
-        secs = self.lib.or_integrate_batch(
-            self._m, n, float(Tstart), float(Tend), _ptr(C), _ptr(RCONST), _ptr(ATOL),
-            _ptr(RTOL), _ptr(ICNTRL), _ptr(RCNTRL), _ptr(Perm), _ptr(ISTAT),
-            _ptr(RSTAT), _ptr(IERR), int(nthreads))
+        if masks is None:
+            secs = self.lib.or_integrate_batch(
+                self._m, n, float(Tstart), float(Tend), _ptr(C), _ptr(RCONST), _ptr(ATOL),
+                _ptr(RTOL), _ptr(ICNTRL), _ptr(RCNTRL), _ptr(Perm), _ptr(ISTAT),
+                _ptr(RSTAT), _ptr(IERR), int(nthreads))
+        else:
+            wv, wj = (self.nvar + 31) // 32, (self.nnz + 31) // 32
+            mk = []
+            for a, w in zip(masks, (wv, wj, wv)):
+                if a is None:
+                    mk.append(None)
+                    continue
+                a = np.ascontiguousarray(a, np.uint32)
+                assert a.shape == (n, w), (a.shape, (n, w))
+                mk.append(a)
+            secs = self.lib.or_integrate_batch_m(
+                self._m, n, float(Tstart), float(Tend), _ptr(C), _ptr(RCONST), _ptr(ATOL),
+                _ptr(RTOL), _ptr(ICNTRL), _ptr(RCNTRL), _ptr(Perm), _ptr(ISTAT),
+                _ptr(RSTAT), _ptr(IERR), int(nthreads),
+                *[None if a is None else _ptr(a) for a in mk])
         return C, ISTAT, RSTAT, IERR, secs

@@ -63,26 +63,53 @@ long or_work_doubles(const or_mech *m)
            m->nvar + m->nvar + m->nreact + m->nfix + 8;
 }
 
+/* Flag k of an auto-reduce mask (32 flags per word, flag k in bit k%32 of
+ * word k/32); a null mask has every flag on. */
+static inline int flag_on(const unsigned *mask, int k)
+{
+    return mask == NULL || ((mask[k >> 5] >> (k & 31)) & 1u) != 0u;
+}
+
 /* Fun_SPLIT (gckpp_Function.F90:2170-5060): rates A(r), then P_VAR and D_VAR
- * sums, then Vdot = P_VAR - D_VAR*V elementwise (:5058). */
+ * sums, then Vdot = P_VAR - D_VAR*V elementwise (:5058).  do_fun: the
+ * IF (DO_FUN(i)) guards of both sums (:3280-5056); a species whose flag is
+ * off keeps the P[i], D[i] passed in. */
+void or_fun_split_m(const or_mech *m, const double *V, const double *F,
+                    const double *RCT, double *Vdot, double *P, double *D,
+                    double *A, const unsigned *do_fun)
+{
+    const double *base[6] = {m->lit, A, RCT, V, F, NULL};
+    for (int r = 0; r < m->nreact; ++r) A[r] = eval_out(&m->rate, base, r);
+    for (int i = 0; i < m->nvar; ++i)
+        if (flag_on(do_fun, i)) P[i] = eval_out(&m->prod, base, i);
+    for (int i = 0; i < m->nvar; ++i)
+        if (flag_on(do_fun, i)) D[i] = eval_out(&m->dest, base, i);
+    for (int i = 0; i < m->nvar; ++i) Vdot[i] = P[i] - D[i] * V[i];
+}
+
 void or_fun_split(const or_mech *m, const double *V, const double *F,
                   const double *RCT, double *Vdot, double *P, double *D,
                   double *A)
 {
-    const double *base[6] = {m->lit, A, RCT, V, F, NULL};
-    for (int r = 0; r < m->nreact; ++r) A[r] = eval_out(&m->rate, base, r);
-    for (int i = 0; i < m->nvar; ++i) P[i] = eval_out(&m->prod, base, i);
-    for (int i = 0; i < m->nvar; ++i) D[i] = eval_out(&m->dest, base, i);
-    for (int i = 0; i < m->nvar; ++i) Vdot[i] = P[i] - D[i] * V[i];
+    or_fun_split_m(m, V, F, RCT, Vdot, P, D, A, NULL);
 }
 
-/* Jac_SP (gckpp_Jacobian.F90:47-20887): B temporaries then JVS sums. */
-void or_jac_sp(const or_mech *m, const double *V, const double *F,
-               const double *RCT, double *JVS, double *B)
+/* Jac_SP (gckpp_Jacobian.F90:47-20887): B temporaries then JVS sums.
+ * do_jvs: the IF (DO_JVS(k)) guards (:3676-20887); an entry whose flag is
+ * off keeps the JVS[k] passed in. */
+void or_jac_sp_m(const or_mech *m, const double *V, const double *F,
+                 const double *RCT, double *JVS, double *B, const unsigned *do_jvs)
 {
     const double *base[6] = {m->lit, NULL, RCT, V, F, B};
     for (int k = 0; k < m->nb; ++k) B[k] = eval_out(&m->jacb, base, k);
-    for (int k = 0; k < m->nnz; ++k) JVS[k] = eval_out(&m->jvs, base, k);
+    for (int k = 0; k < m->nnz; ++k)
+        if (flag_on(do_jvs, k)) JVS[k] = eval_out(&m->jvs, base, k);
+}
+
+void or_jac_sp(const or_mech *m, const double *V, const double *F,
+               const double *RCT, double *JVS, double *B)
+{
+    or_jac_sp_m(m, V, F, RCT, JVS, B, NULL);
 }
 
 /* KppDecomp (gckpp_LinearAlgebra.F90:46-84): in-place unpivoted sparse LU in
@@ -107,20 +134,29 @@ void or_decomp(const or_mech *m, double *JVS, double *W, int *ier)
 }
 
 /* KppSolve (gckpp_LinearAlgebra.F90:645-2311): forward L (unit diagonal) in
- * row order, then backward U rows NVAR..1 with one divide per row. */
-void or_solve(const or_mech *m, const double *JVS, double *X)
+ * row order, then backward U rows NVAR..1 with one divide per row.  do_slv:
+ * the IF (DO_SLV(i)) guard of each row's statement (:653-2309); a row whose
+ * flag is off leaves X[i] as it is. */
+void or_solve_m(const or_mech *m, const double *JVS, double *X, const unsigned *do_slv)
 {
     const int *crow = m->crow, *icol = m->icol, *diag = m->diag;
     for (int i = 0; i < m->nvar; ++i) {
+        if (!flag_on(do_slv, i)) continue;
         double x = X[i];
         for (int j = crow[i]; j < diag[i]; ++j) x = x - JVS[j] * X[icol[j]];
         X[i] = x;
     }
     for (int i = m->nvar - 1; i >= 0; --i) {
+        if (!flag_on(do_slv, i)) continue;
         double x = X[i];
         for (int j = diag[i] + 1; j < crow[i + 1]; ++j) x = x - JVS[j] * X[icol[j]];
         X[i] = x / JVS[diag[i]];
     }
+}
+
+void or_solve(const or_mech *m, const double *JVS, double *X)
+{
+    or_solve_m(m, JVS, X, NULL);
 }
 
 /* KppSolveTR (gckpp_LinearAlgebra.F90:2327-3701): (LU)^T XX = X.  Phase 1
@@ -183,6 +219,21 @@ void or_solve_cell_rec(const or_mech *m, double *Y, const double *FIX,
                        int *ISTAT, double *RSTAT, int *IERR, double *work,
                        double *Hrec, int maxrec, int *nrec)
 {
+    or_solve_cell_m(m, Y, FIX, RCONST, Tstart, Tend, AbsTol, RelTol, ICNTRL_U, RCNTRL_U,
+                    ISTAT, RSTAT, IERR, work, Hrec, maxrec, nrec, NULL, NULL, NULL);
+}
+
+/* The cell solve with its auto-reduce masks (null: every flag on).  The
+ * workspace the guarded statements leave alone -- P_VAR, D_VAR (wP, wD) and
+ * Jac0 -- holds zeros from the cell's start. */
+void or_solve_cell_m(const or_mech *m, double *Y, const double *FIX,
+                     const double *RCONST, double Tstart, double Tend,
+                     const double *AbsTol, const double *RelTol,
+                     const int *ICNTRL_U, const double *RCNTRL_U,
+                     int *ISTAT, double *RSTAT, int *IERR, double *work,
+                     double *Hrec, int maxrec, int *nrec, const unsigned *do_fun,
+                     const unsigned *do_jvs, const unsigned *do_slv)
+{
     if (Hrec) *nrec = 0;
     static const double ros_A[6] = {0.0, 2.0, 0.0, 2.0, 0.0, 1.0};
     static const double ros_C[6] = {4.0, 1.0, -1.0, 1.0, -1.0, -(8.0 / 3.0)};
@@ -214,6 +265,9 @@ void or_solve_cell_rec(const or_mech *m, double *Y, const double *FIX,
     double *A = wD + NVAR;
     double *B = A + m->nreact;
     double *W = B + m->nb;
+    memset(Jac0, 0, sizeof(double) * (size_t)NNZ);
+    memset(wP, 0, sizeof(double) * (size_t)NVAR);
+    memset(wD, 0, sizeof(double) * (size_t)NVAR);
 
     int Nsteps = 0;
     *IERR = 0;
@@ -250,15 +304,15 @@ void or_solve_cell_rec(const or_mech *m, double *Y, const double *FIX,
         if (((T + 0.1 * H) == T) || (H <= ROUNDOFF)) { *IERR = -7; return; }
         H = fmin_f(H, fabs(Tend - T));
 
-        or_fun_split(m, Y, FIX, RCONST, Fcn0, wP, wD, A);
+        or_fun_split_m(m, Y, FIX, RCONST, Fcn0, wP, wD, A, do_fun);
         ISTAT[0] += 1;
         if (!Autonomous) {
             double Delta = sqrt(ROUNDOFF) * fmax_f(1.0e-6, fabs(T));
-            or_fun_split(m, Y, FIX, RCONST, dFdT, wP, wD, A);
+            or_fun_split_m(m, Y, FIX, RCONST, dFdT, wP, wD, A, do_fun);
             ISTAT[0] += 1;
             for (int k = 0; k < NVAR; ++k) dFdT[k] = (dFdT[k] - Fcn0[k]) / Delta;
         }
-        or_jac_sp(m, Y, FIX, RCONST, Jac0, B);
+        or_jac_sp_m(m, Y, FIX, RCONST, Jac0, B, do_jvs);
         ISTAT[1] += 1;
 
         for (;;) { /* UntilAccepted */
@@ -293,7 +347,7 @@ void or_solve_cell_rec(const or_mech *m, double *Y, const double *FIX,
                         for (int k = 0; k < NVAR; ++k)
                             Ynew[k] = Ynew[k] + aa * Kw[NVAR * (j - 1) + k];
                     }
-                    or_fun_split(m, Ynew, FIX, RCONST, Fcn, wP, wD, A);
+                    or_fun_split_m(m, Ynew, FIX, RCONST, Fcn, wP, wD, A, do_fun);
                     ISTAT[0] += 1;
                 }
                 for (int k = 0; k < NVAR; ++k) Ks[k] = Fcn[k];
@@ -306,7 +360,7 @@ void or_solve_cell_rec(const or_mech *m, double *Y, const double *FIX,
                     double HG = (Dir * H) * ros_Gamma[istage - 1];
                     for (int k = 0; k < NVAR; ++k) Ks[k] = Ks[k] + HG * dFdT[k];
                 }
-                or_solve(m, Ghimj, Ks);
+                or_solve_m(m, Ghimj, Ks, do_slv);
                 ISTAT[6] += 1;
             }
 
@@ -442,6 +496,23 @@ double or_integrate_batch(const or_mech *m, int nCell, double Tstart,
                           const int *Perm, int *ISTAT, double *RSTAT,
                           int *IERR, int nthreads)
 {
+    return or_integrate_batch_m(m, nCell, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL,
+                                Perm, ISTAT, RSTAT, IERR, nthreads, NULL, NULL, NULL);
+}
+
+/* As or_integrate_batch with per-cell auto-reduce masks (column idx of the
+ * caller's arrays has its DO_FUN words at do_fun + idx*ceil(NVAR/32), DO_JVS
+ * at do_jvs + idx*ceil(LU_NONZERO/32), DO_SLV at do_slv + idx*ceil(NVAR/32);
+ * a null array: every flag on). */
+double or_integrate_batch_m(const or_mech *m, int nCell, double Tstart,
+                            double Tend, double *C, const double *RCONST,
+                            const double *ATOL, const double *RTOL,
+                            const int *ICNTRL, const double *RCNTRL,
+                            const int *Perm, int *ISTAT, double *RSTAT,
+                            int *IERR, int nthreads, const unsigned *do_fun,
+                            const unsigned *do_jvs, const unsigned *do_slv)
+{
+    const long wv = (m->nvar + 31) / 32, wj = (m->nnz + 31) / 32;
     if (nCell <= 0) return 0.0;
     const int nspec = m->nvar + m->nfix;
     const long nw = or_work_doubles(m);
@@ -460,9 +531,12 @@ double or_integrate_batch(const or_mech *m, int nCell, double Tstart,
             memcpy(Yl, Cc, sizeof(double) * m->nvar);
             memcpy(FIXl, Cc + m->nvar, sizeof(double) * m->nfix);
             memcpy(RCl, RCONST + idx * m->nreact, sizeof(double) * m->nreact);
-            or_solve_cell(m, Yl, FIXl, RCl, Tstart, Tend, ATOL, RTOL,
-                          ICNTRL + idx * 20, RCNTRL + idx * 20,
-                          ISTAT + idx * 8, RSTAT + idx * 3, IERR + idx, work);
+            or_solve_cell_m(m, Yl, FIXl, RCl, Tstart, Tend, ATOL, RTOL,
+                            ICNTRL + idx * 20, RCNTRL + idx * 20,
+                            ISTAT + idx * 8, RSTAT + idx * 3, IERR + idx, work,
+                            NULL, 0, NULL, do_fun ? do_fun + idx * wv : NULL,
+                            do_jvs ? do_jvs + idx * wj : NULL,
+                            do_slv ? do_slv + idx * wv : NULL);
             memcpy(Cc, Yl, sizeof(double) * m->nvar);
         }
         free(work);

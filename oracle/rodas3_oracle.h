@@ -55,6 +55,16 @@ void or_jac_sp(const or_mech *m, const double *V, const double *F,
                const double *RCT, double *JVS, double *B);
 void or_decomp(const or_mech *m, double *JVS, double *W, int *ier);
 void or_solve(const or_mech *m, const double *JVS, double *X);
+
+/* The same with the auto-reduce guards of the generated code: flag k of a
+ * mask is bit k%32 of word k/32, a null mask has every flag on; a guarded
+ * statement whose flag is off leaves its output as passed in. */
+void or_fun_split_m(const or_mech *m, const double *V, const double *F,
+                    const double *RCT, double *Vdot, double *P, double *D,
+                    double *A, const unsigned *do_fun);
+void or_jac_sp_m(const or_mech *m, const double *V, const double *F,
+                 const double *RCT, double *JVS, double *B, const unsigned *do_jvs);
+void or_solve_m(const or_mech *m, const double *JVS, double *X, const unsigned *do_slv);
 void or_solve_tr(const or_mech *m, const double *JVS, const double *X,
                  double *XX);
 
@@ -74,6 +84,16 @@ void or_solve_cell_rec(const or_mech *m, double *Y, const double *FIX,
                        int *ISTAT, double *RSTAT, int *IERR, double *work,
                        double *Hrec, int maxrec, int *nrec);
 
+/* Integrate_Cell_S with per-cell auto-reduce masks (null: every flag on);
+ * P_VAR, D_VAR and Jac0 hold zeros from the cell's start. */
+void or_solve_cell_m(const or_mech *m, double *Y, const double *FIX,
+                     const double *RCONST, double Tstart, double Tend,
+                     const double *ATOL, const double *RTOL,
+                     const int *ICNTRL_U, const double *RCNTRL_U,
+                     int *ISTAT, double *RSTAT, int *IERR, double *work,
+                     double *Hrec, int maxrec, int *nrec, const unsigned *do_fun,
+                     const unsigned *do_jvs, const unsigned *do_slv);
+
 /* Accepted steps with given step sizes Hs[0..nsteps) (frozen step-size
  * control; autonomous).  *ier = 0 or 1 + the index of a singular step. */
 void or_replay_cell(const or_mech *m, double *Y, const double *FIX,
@@ -88,6 +108,15 @@ double or_integrate_batch(const or_mech *m, int nCell, double Tstart,
                           const int *ICNTRL, const double *RCNTRL,
                           const int *Perm, int *ISTAT, double *RSTAT,
                           int *IERR, int nthreads);
+
+/* As or_integrate_batch with per-cell masks (ceil(n/32) words per cell). */
+double or_integrate_batch_m(const or_mech *m, int nCell, double Tstart,
+                            double Tend, double *C, const double *RCONST,
+                            const double *ATOL, const double *RTOL,
+                            const int *ICNTRL, const double *RCNTRL,
+                            const int *Perm, int *ISTAT, double *RSTAT,
+                            int *IERR, int nthreads, const unsigned *do_fun,
+                            const unsigned *do_jvs, const unsigned *do_slv);
 
 #ifdef __cplusplus
 }

@@ -322,3 +322,88 @@ def test_step_control_known_answer():
         assert IE[0] == 1
         assert list(IS[0]) == ist, mode
         assert C1[0, 0] == y and list(RS[0]) == rst, mode
+
+
+# ---- auto-reduce masks (DO_FUN / DO_JVS / DO_SLV) --------------------------
+def test_masked_functions_guard_their_statements(mech, oracle_cr):
+    """Each guard skips exactly its statement: a species whose DO_FUN flag is
+    off keeps the P/D passed in (zero: Vdot = 0), an entry whose DO_JVS flag
+    is off keeps the JVS passed in, every other output equals the unmasked
+    routine's bit for bit; all flags on is the unmasked routine."""
+    o, n, nnz = oracle_cr, mech.nvar, mech.nnz
+    rng = np.random.default_rng(4)
+    V = np.exp(rng.uniform(np.log(1e2), np.log(1e10), n))
+    F = np.array([3e17, 1.9e19, 5e18])
+    R = np.exp(rng.uniform(np.log(1e-15), np.log(1e-10), mech.nreact))
+    fon = rng.uniform(size=n) < 0.6
+    vd, P, D, _ = o.fun_split(V, F, R)
+    vm, Pm, Dm, _ = o.fun_split(V, F, R, do_fun=o.mask_words(fon))
+    assert np.array_equal(vm[fon], vd[fon]) and np.all(vm[~fon] == 0.0)
+    P0, D0 = np.full(n, 7.0), np.full(n, 0.5)
+    vs, Ps, Ds, _ = o.fun_split(V, F, R, do_fun=o.mask_words(fon), P0=P0, D0=D0)
+    assert np.all(Ps[~fon] == 7.0) and np.all(Ds[~fon] == 0.5)
+    assert np.array_equal(vs[~fon], 7.0 - 0.5 * V[~fon])
+    assert np.array_equal(o.fun_split(V, F, R, do_fun=o.mask_words(np.ones(n, bool)))[0], vd)
+    jon = rng.uniform(size=nnz) < 0.6
+    j = o.jac_sp(V, F, R)
+    jm = o.jac_sp(V, F, R, do_jvs=o.mask_words(jon), JVS0=np.full(nnz, -3.25))
+    assert np.array_equal(jm[jon], j[jon]) and np.all(jm[~jon] == -3.25)
+
+
+def test_masked_solve_equals_dense_substitution(mech, oracle_cr, prepared):
+    """KppSolve with DO_SLV flags off for some rows: dense forward and backward
+    substitution (unit L, then U) that leaves exactly those rows' x alone."""
+    lu, _ = oracle_cr.decomp(prepared)
+    LU = _dense(mech, lu)
+    L, U = np.tril(LU, -1), np.triu(LU)
+    rng = np.random.default_rng(6)
+    on = rng.uniform(size=mech.nvar) < 0.7
+    b = rng.normal(size=mech.nvar)
+    x = b.copy()
+    for i in range(mech.nvar):
+        if on[i]:
+            x[i] = x[i] - L[i, :i] @ x[:i]
+    for i in range(mech.nvar - 1, -1, -1):
+        if on[i]:
+            x[i] = (x[i] - U[i, i + 1:] @ x[i + 1:]) / U[i, i]
+    got = oracle_cr.solve(lu, b, do_slv=oracle_cr.mask_words(on))
+    assert np.allclose(got, x, rtol=1e-10, atol=1e-12 * np.abs(x).max())
+    assert np.array_equal(got[~on], b[~on])
+
+
+def test_masked_species_frozen_robertson_against_radau():
+    """Robertson's kinetics with species C switched off the way auto-reduce
+    switches off an inactive species -- its DO_FUN and DO_SLV flags and the
+    DO_JVS flags of its Jacobian row and column -- integrate the reduced
+    system with C held at its initial value: C comes back unchanged and A, B
+    agree with SciPy's Radau on A' = -k1 A + k3 B C0, B' = k1 A - k3 B C0 - k2 B^2.
+    A batch with every flag on equals the unmasked solve bit for bit."""
+    rx = [synthetic.Reaction({0: 1}, {1: 1.0}),
+          synthetic.Reaction({1: 2}, {1: 1.0, 2: 1.0}),
+          synthetic.Reaction({1: 1, 2: 1}, {0: 1.0, 2: 1.0})]
+    ir = synthetic.build_ir("robertson", 3, 1, rx, reorder=False)
+    o = Oracle(ir, pow_mode="cr")
+    k = np.array([[0.04, 3e7, 1e4]])
+    C0 = 0.05
+    C = np.array([[1.0, 0.0, C0, 0.0]])
+    I = np.zeros((1, 20), np.int32)
+    I[0, 0], I[0, 2] = 1, 4
+    args = (0.0, 40.0, C, k, np.full(3, 1e-14), np.full(3, 1e-7), I, np.zeros((1, 20)))
+    rows = np.repeat(np.arange(3), np.diff(ir["crow"]))
+    cols = np.asarray(ir["icol"])
+    frozen = np.array([False, False, True])
+    masks = (o.mask_words(~frozen)[None], o.mask_words(~(frozen[rows] | frozen[cols]))[None],
+             o.mask_words(~frozen)[None])
+    C1, IS, RS, IE, _ = o.integrate_batch(*args, masks=masks)
+    assert IE[0] == 1 and C1[0, 2] == C0
+
+    def f(t, y):
+        a, b = y
+        return [-0.04 * a + 1e4 * b * C0, 0.04 * a - 1e4 * b * C0 - 3e7 * b * b]
+
+    ref = solve_ivp(f, (0.0, 40.0), [1.0, 0.0], method="Radau", rtol=1e-11, atol=1e-16).y[:, -1]
+    assert np.allclose(C1[0, :2], ref, rtol=2e-5, atol=0)
+    full = o.integrate_batch(*args)
+    on = tuple(np.full_like(m_, 0xFFFFFFFF) for m_ in masks)
+    alln = o.integrate_batch(*args, masks=on)
+    assert all(np.array_equal(a, b) for a, b in zip(full[:4], alln[:4]))

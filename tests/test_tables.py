@@ -60,6 +60,9 @@ class TableMachine:
         self.ws[self.E["ONE"]] = 1.0
         self.ws[self.E["NZERO"]] = -0.0
         self.lds = np.full((seg + 1, lanes), np.nan)
+        # auto-reduce flags per lane ([lanes, n] booleans) for the masked
+        # kernels' guards; None: unmasked kernels
+        self.do_fun = self.do_jvs = self.do_slv = None
 
     # workspace access by byte offset e*512 (a lane's column)
     def ld(self, soff):
@@ -113,6 +116,9 @@ class TableMachine:
                 P, p = self.sum_terms(w, p, nP)
             if nD:
                 D, p = self.sum_dterms(w, p, nD)
+            if self.do_fun is not None:
+                on = self.do_fun[:, oo // 512]
+                P, D = np.where(on, P, 0.0), np.where(on, D, 0.0)
             self.st(out + oo, P - D * self.ld(vo))
         assert p == len(w)
 
@@ -146,11 +152,13 @@ class TableMachine:
                     cx = _coef(w[p + 2], w[p + 3]) * self.ld(soff)
                     acc = cx if fl & 1 else acc + cx
                     if fl & 2:
+                        if self.do_jvs is not None:
+                            acc = np.where(self.do_jvs[:, fl >> 14], acc, 0.0)
                         g = -acc
                         if fl & 4:
                             g = g + ghinv
                             sing |= np.abs(g) < 2.2250738585072014e-308
-                        self.lds[fl >> 8] = g
+                        self.lds[(fl >> 8) & 63] = g
                     p += 4
             elif code in (2, 3):
                 if code == 2:
@@ -176,7 +184,10 @@ class TableMachine:
                     p += 2
         return sing
 
-    def solve_rows(self, rows, div):
+    def _slv_on(self, xi, xbase):
+        return self.do_slv[:, xi // 512 - xbase]
+
+    def solve_rows(self, rows, div, xbase=None):
         w, nrows = rows
         p = 0
         for _ in range(nrows):
@@ -184,15 +195,18 @@ class TableMachine:
             assert n % 4 == 0
             p += 4
             x = self.ld(xi).copy()
+            x0 = x
             for _ in range(n):
                 x = x - self.ld(w[p]) * self.ld(w[p + 1])
                 p += 2
             if div:
                 x = x / self.ld(dg)
+            if xbase is not None and self.do_slv is not None:
+                x = np.where(self._slv_on(xi, xbase), x, x0)
             self.st(xi, x)
         assert p == len(w)
 
-    def solve_bwd_lds(self, rows):
+    def solve_bwd_lds(self, rows, xbase):
         """Backward rows with the last seg columns' x_j in LDS (bwd_lds)."""
         w, nrows = rows
         lds = lambda off: self.lds[off // 512]
@@ -205,6 +219,7 @@ class TableMachine:
             assert nc % 4 == 0 and nh % 4 == 0
             p += 4
             x = self.ld(xi).copy()
+            x0 = x
             for _ in range(nc):
                 x = x - self.ld(w[p]) * self.ld(w[p + 1])
                 p += 2
@@ -212,6 +227,8 @@ class TableMachine:
                 x = x - self.ld(w[p]) * lds(w[p + 1])
                 p += 2
             x = x / self.ld(dg)
+            if self.do_slv is not None:
+                x = np.where(self._slv_on(xi, xbase), x, x0)
             self.st(xi, x)
             if li >= 0:
                 assert li // 512 < self.seg
@@ -219,8 +236,9 @@ class TableMachine:
         assert p == len(w)
 
     def solve(self, s):
-        self.solve_rows(self.fwd[s], False)
-        self.solve_bwd_lds(self.bwd[s])
+        xbase = self.E["K"] + s * self.ir["nvar"]
+        self.solve_rows(self.fwd[s], False, xbase)
+        self.solve_bwd_lds(self.bwd[s], xbase)
 
     # -- adjoint routines, as adj_* in gc_rodas3.hip ---------------------------
     ROS_A = (0.0, 2.0, 0.0, 2.0, 0.0, 1.0)
@@ -516,6 +534,54 @@ def _check(mech, orc, seg, seed):
     for c in range(LANES):
         vdot_n, *_ = orc.fun_split(Vn[c], F[c], R[c])
         assert _same(m.vec("K", 1)[:, c], vdot_n)
+
+
+@pytest.mark.parametrize("seg", (16, 32))
+@pytest.mark.parametrize("name", ["fullchem", "stress500"])
+def test_masked_tables_match_oracle(name, seg):
+    """The masked kernels' guards, interpreted over the tables with random
+    per-lane auto-reduce flags (DO_FUN, DO_JVS, DO_SLV), equal the oracle's
+    guarded Fun_SPLIT (P/D starting at zero), Jac_SP (Jac0 starting at
+    zero), KppDecomp and KppSolve bit for bit."""
+    from gcchem_amd import mechanism
+    from oracle import Oracle
+    mech = mechanism.load(name)
+    orc = Oracle(mech.ir, pow_mode="cr")
+    ir = mech.ir
+    n, nnz = ir["nvar"], ir["nnz"]
+    V, F, R = _cells(mech, LANES, 11)
+    rng = np.random.default_rng(12)
+    m = TableMachine(ir, LANES, seg)
+    m.do_fun = rng.uniform(size=(LANES, n)) < 0.7
+    m.do_jvs = rng.uniform(size=(LANES, nnz)) < 0.8
+    m.do_slv = rng.uniform(size=(LANES, n)) < 0.7
+    m.do_fun[0], m.do_jvs[0], m.do_slv[0] = True, True, True    # lane 0: every flag on
+    _load(m, V, F, R)
+    m.fun(False, m.E["FCN0"] * 512)
+    ghinv = np.array([1.0e3, 3.7, 0.05, 1.0e-6])[:LANES]
+    m.jac_lu(ghinv)
+    lu_dev = m.ws[m.E["LU"]:m.E["LU"] + nnz]
+    rhs = np.stack([np.sin(np.arange(n) + s) * 10.0 ** (s - 2) for s in range(4)])
+    for s in range(4):
+        m.ws[m.E["K"] + s * n:m.E["K"] + (s + 1) * n] = rhs[s][:, None]
+        m.solve(s)
+    diag = np.asarray(ir["diag"])
+    W = Oracle.mask_words
+    for c in range(LANES):
+        vdot, *_ = orc.fun_split(V[c], F[c], R[c], do_fun=W(m.do_fun[c]))
+        assert _same(m.vec("FCN0")[:, c], vdot)
+        if c == 0:
+            assert _same(vdot, orc.fun_split(V[c], F[c], R[c])[0])
+        assert np.all(vdot[~m.do_fun[c]] == 0.0)
+        jvs = orc.jac_sp(V[c], F[c], R[c], do_jvs=W(m.do_jvs[c]))
+        g = -jvs
+        g[diag] = g[diag] + ghinv[c]
+        lu, ier = orc.decomp(g)
+        assert ier == 0 and _same(lu_dev[:, c], lu)
+        for s in range(4):
+            x = orc.solve(lu, rhs[s], do_slv=W(m.do_slv[c]))
+            assert _same(m.vec("K", s)[:, c], x)
+            assert _same(x[~m.do_slv[c]], rhs[s][~m.do_slv[c]])
 
 
 def test_padding_is_exact_noop():
