@@ -283,76 +283,97 @@ STRESS_SIZES = (1000, 4000, 16000, 32000, 64000, 100000, 150000, 200000)
 def bench_stress(args):
     """Synthetic stress mechanism (500 species, 1,500 reactions, ~3% LU fill)
     in eight ragged batches of 1k-200k cells, dt 600 s, each batch in its
-    own launch order (no previous step to sort by): all batches in ONE
-    persistent launch (gc_amd_integrate_batches_device, one work queue
-    through the batches, progress lines as cells finish), the plain kernel
-    on each batch in turn for the ratio, and the adjoint (dL/dC_final = 1) of
-    the two smallest batches as one recording launch plus one sweep over both,
-    against the forward launch of the same two batches."""
+    own launch order (no previous step to sort by).  Legs (--stress-legs):
+    fwd, all batches in ONE persistent launch (gc_amd_integrate_batches_device,
+    one work queue through the batches, progress lines as cells finish);
+    plain, the plain kernel on each batch in turn, for the ratio and an
+    equality check; adj, the discrete adjoint (dL/dC_final = 1) of all eight
+    batches as ONE recording launch (record pool sized from the forward
+    launch's accepted steps) and ONE persistent sweep, against the forward
+    launch."""
     import numpy as np
     import torch
     from gcchem_amd import adjoint, inputs, mechanism, solver
     mech = mechanism.load("stress500")
+    legs = set(args.stress_legs.split(","))
     sizes = [int(x) for x in args.stress_sizes.split(",")] if args.stress_sizes else STRESS_SIZES
-    bs = [solver.Batch.from_numpy(inputs.stress_cells(mech, n, seed=40 + i))
+    bs = [solver.Batch.from_numpy(inputs.stress_cells(mech, n, seed=40 + i,
+                                                      max_steps=args.stress_max_steps))
           for i, n in enumerate(sizes)]
     ncell = sum(sizes)
-    log(f"stress: {len(sizes)} batches {sizes} generated ({ncell} cells)")
-    step = [0.05]
+    log(f"stress: {len(sizes)} batches {sizes} generated ({ncell} cells), Max_no_steps "
+        f"{args.stress_max_steps or 200000}")
 
-    def progress(nb, done, total):
-        if done >= step[0] * total or nb == len(sizes):
-            log(f"one launch: {done}/{total} cells finished, {nb}/{len(sizes)} batches")
-            while step[0] * total <= done and step[0] <= 1.0:
-                step[0] += 0.05
+    def reporter(what):
+        step = [0.05]
 
-    outs, t_one = solver.solve_batches(bs, mech="stress500", progress=progress)
+        def progress(nb, done, total):
+            if done >= step[0] * total or nb == len(sizes):
+                log(f"{what}: {done}/{total} cells finished, {nb}/{len(sizes)} batches")
+                while step[0] * total <= done and step[0] <= 1.0:
+                    step[0] += 0.05
+        return progress
+
+    outs, t_one = solver.solve_batches(bs, mech="stress500", progress=reporter("one launch"))
     torch.cuda.synchronize()
     log(f"all {len(sizes)} batches in one persistent launch: {t_one:.2f} s")
     per, total_q = [], 0.0
     for b, r, n in zip(bs, outs, sizes):
-        ok = float((r.IERR == 1).float().mean().item())
-        nst = float(r.ISTAT[:, 2].float().mean())
-        rq = solver.solve(b, kernel="plain", mech="stress500")
-        torch.cuda.synchronize()
-        same = all(torch.equal(x, y) for x, y in ((r.C, rq.C), (r.ISTAT, rq.ISTAT),
-                                                  (r.RSTAT, rq.RSTAT), (r.IERR, rq.IERR)))
-        log(f"stress batch {n}: plain {rq.t_kernel:.2f} s, mean steps {nst:.0f}, IERR=1 "
-            f"fraction {ok:.4f} (the rest -6, for the caller's CPU retry), one launch == "
-            f"plain: {same}")
-        per.append({"cells": n, "plain_s": round(rq.t_kernel, 3), "mean_steps": round(nst, 1),
-                    "ierr_ok_frac": ok, "one_launch_equals_plain": same})
-        total_q += rq.t_kernel
-        del rq
+        ist = r.ISTAT.cpu().numpy()
+        row = {"cells": n, "mean_steps": round(float(ist[:, 2].mean()), 1),
+               "max_steps": int(ist[:, 2].max()),
+               "ierr_ok_frac": float((r.IERR == 1).float().mean().item())}
+        if "plain" in legs:
+            rq = solver.solve(b, kernel="plain", mech="stress500")
+            torch.cuda.synchronize()
+            row["plain_s"] = round(rq.t_kernel, 3)
+            row["one_launch_equals_plain"] = all(
+                torch.equal(x, y) for x, y in ((r.C, rq.C), (r.ISTAT, rq.ISTAT),
+                                               (r.RSTAT, rq.RSTAT), (r.IERR, rq.IERR)))
+            total_q += rq.t_kernel
+            del rq
+        log(f"stress batch {n}: {row}")
+        per.append(row)
     adj = None
-    if not args.no_adjoint:
-        two = bs[:2]
-        _, t_fwd = solver.solve_batches(two, mech="stress500")
-        outs2, rec = adjoint.record_solve_batches(two, mech="stress500")
+    if "adj" in legs:
+        cap = int(sum(int(r.ISTAT[:, 3].sum().item()) for r in outs))  # accepted steps
+        outs2, rec = adjoint.record_solve_batches(bs, mech="stress500", cap=cap,
+                                                  progress=reporter("recording launch"))
         torch.cuda.synchronize()
-        log(f"adjoint, batches {sizes[:2]}: forward {t_fwd:.2f} s, recording {rec.t_kernel:.2f} s")
-        lams = [torch.ones_like(o.C) for o in outs2]
-        grads, t_sweep = adjoint.adjoint_sweep_batches(two, rec, lams, mech="stress500")
+        same = all(torch.equal(a.C, b.C) and torch.equal(a.ISTAT, b.ISTAT)
+                   for a, b in zip(outs, outs2))
+        log(f"adjoint: recording launch {rec.t_kernel:.2f} s ({rec.used} records, every "
+            f"{rec.every}), results == forward launch: {same}")
+        # the sweep runs each batch's costly cells first (descending accepted steps)
+        perms = [inputs.cost_order(o.ISTAT[:, 3].cpu().numpy()) for o in outs2]
+        del outs2
+        lams = [torch.ones_like(b.C) for b in bs]
+        grads, t_sweep = adjoint.adjoint_sweep_batches(bs, rec, lams, mech="stress500",
+                                                       progress=reporter("sweep"), perms=perms)
         torch.cuda.synchronize()
-        ratio = (rec.t_kernel + t_sweep) / t_fwd
-        log(f"adjoint, batches {sizes[:2]}: sweep {t_sweep:.2f} s, adj/fwd {ratio:.2f}")
-        adj = {"batches": sizes[:2], "forward_s": round(t_fwd, 3), "record_s": round(rec.t_kernel, 3),
-               "sweep_s": round(t_sweep, 3), "records": rec.used, "adj_over_fwd": round(ratio, 3),
-               "ierr_ok_frac": float(np.mean([float((g[2] == 1).float().mean()) for g in grads]))}
-    return {
+        ratio = (rec.t_kernel + t_sweep) / t_one
+        ok = float(np.mean([float((g[2] == 1).float().mean()) for g in grads]))
+        log(f"adjoint: sweep {t_sweep:.2f} s, adj/fwd {ratio:.3f}, IERR=1 fraction {ok:.4f}")
+        adj = {"batches": list(sizes), "forward_s": round(t_one, 3),
+               "record_s": round(rec.t_kernel, 3), "sweep_s": round(t_sweep, 3),
+               "records": rec.used, "every": rec.every, "record_equals_forward": same,
+               "adj_over_fwd": round(ratio, 3), "ierr_ok_frac": ok, "one_launch_each": True}
+    line = {
         "metric": METRIC, "value": round(ncell / t_one, 1), "unit": "cells/s", "n_gpus": 1,
         "steps": 1, "warmup": 0, "ms_per_step": round(t_one * 1e3, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"synthetic stress mechanism 500 species x 1500 reactions, "
                                f"{len(sizes)} ragged batches {sizes} in one persistent launch, "
-                               f"dt 600 s, Max_no_steps 1000 (longer cells -> IERR -6, CPU retry)",
+                               f"dt 600 s, Max_no_steps {args.stress_max_steps or 200000}",
                    "cells": ncell, "one_launch": True},
-        "plain_cells_per_s": round(ncell / total_q, 1),
-        "speedup_vs_plain": round(total_q / t_one, 3),
         "batches": per, "adjoint": adj,
         "adj_over_fwd": adj["adj_over_fwd"] if adj else None,
         "roofline": None, "cpu_baseline": None,
     }
+    if "plain" in legs:
+        line["plain_cells_per_s"] = round(ncell / total_q, 1)
+        line["speedup_vs_plain"] = round(total_q / t_one, 3)
+    return line
 
 
 def main():
@@ -367,6 +388,10 @@ def main():
                     help="4x5 (238,464 cells; the headline line), 2x2.5 (943,488 cells, "
                          "chunked graph) or stress (synthetic 500x1500 mechanism, ragged batches)")
     ap.add_argument("--stress-sizes", default="", help="comma-separated batch sizes (stress)")
+    ap.add_argument("--stress-legs", default="fwd,plain,adj",
+                    help="stress legs: fwd (always), plain, adj")
+    ap.add_argument("--stress-max-steps", type=int, default=1000,
+                    help="ICNTRL(4) of the stress cells (0: the integrator's 200,000)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB elementwise copy first (counter calibration)")
     args = ap.parse_args()

@@ -233,20 +233,26 @@ def record_solve_batches(batches: list[Batch], mech: str = "fullchem", every: in
 
 
 def adjoint_sweep_batches(batches: list[Batch], rec: Records, lams: list[torch.Tensor],
-                          mech: str = "fullchem", progress=None):
+                          mech: str = "fullchem", progress=None, perms=None):
     """Reverse sweep of a record_solve_batches recording in one launch: per
     batch (dL/dC0, dL/dRCONST, IERR), and the kernel seconds; progress as in
-    record_solve_batches."""
+    record_solve_batches.  perms (one 1-based launch order per batch, e.g.
+    inputs.cost_order of the recording's accepted steps, so that each wave's
+    64 cells have alike costs) replaces the batches' own Perm."""
     lib = _lib.load(mech)
     outs, descs = [], []
-    for b, lam in zip(batches, lams):
+    if perms is None:
+        perms = [b.Perm for b in batches]
+    perms = [torch.as_tensor(p, device=b.C.device).to(torch.int32).contiguous()
+             for b, p in zip(batches, perms)]
+    for b, lam, perm in zip(batches, lams, perms):
         lam = lam.to(device=b.C.device, dtype=torch.float64).contiguous()
         gC, gR = torch.empty_like(b.C), torch.empty_like(b.RCONST)
         ierr = torch.empty((b.n,), dtype=torch.int32, device=b.C.device)
         outs.append((gC, gR, ierr, lam))
         p = lambda t: t.data_ptr()
         descs.append(_lib.GcAmdAdjBatch(b.n, float(b.Tstart), float(b.Tend), p(b.C), p(b.RCONST),
-                                        p(b.ICNTRL), p(b.Perm), p(lam), p(gC), p(gR), p(ierr)))
+                                        p(b.ICNTRL), p(perm), p(lam), p(gC), p(gR), p(ierr)))
     arr = (_lib.GcAmdAdjBatch * len(descs))(*descs)
     cb = _lib.PROGRESS_FN(lambda nb, c, t, u: progress(nb, c, t)) if progress else None
     tk = ctypes.c_double(0.0)

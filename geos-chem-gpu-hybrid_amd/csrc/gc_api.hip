@@ -791,8 +791,9 @@ int adjoint_sweep(DevCtx& c, std::vector<GcAdjDesc>& h, hipStream_t st,
   const int nb = (int)h.size();
   const Tuning t = current_tuning();
   const GcVariantInfo vi = variant_info(t.variant);
-  // adjoint block + the segment buffers (`every` start states and step sizes)
-  const long slot_doubles = vi.adj_doubles_per_slot + (long)h[0].a.every * (vi.nvar + 1);
+  // adjoint block + the segment buffers (`every` start states, step sizes
+  // and stage vectors)
+  const long slot_doubles = vi.adj_doubles_per_slot + (long)h[0].a.every * (5L * vi.nvar + 1);
   DoneCounters done;
   if (progress && done.init(nb)) return -2;
   unsigned long long q0 = 0;

@@ -1452,8 +1452,8 @@ __device__ void adj_step(const Blk& b, double Dir, double H, bool clamp) {
 // One lane per cell in Perm order: walks the cell's records from the last
 // to the first one segment at a time (the steps back to a checkpoint),
 // replays the segment forward from its checkpoint to regain each step's start
-// state, then recomputes each step's factors and stages and sweeps the adjoint
-// through it, last step first.
+// state and stage vectors, then, last step first, recomputes each step's
+// factors and sweeps the adjoint through it.
 namespace {
 
 // The reverse sweep of one cell: launch position ic of batch `a`, whose cells
@@ -1488,10 +1488,12 @@ __device__ __forceinline__ void adjoint_cell(const GcAdjArgs& a, unsigned q0, un
   const double Dir = (a.Tend >= a.Tstart) ? 1.0 : -1.0;
   int ist[8];
   int ierr = 1;
-  // segment buffers after the adjoint block: start states, step sizes, records
+  // segment buffers after the adjoint block: start states, step sizes (last
+  // step first) and stage vectors of the segment's steps
   const int m = a.every;
   const int O_SEG = MECH_E_ADJ_TOTAL * 512;
   const int O_SEGH = O_SEG + m * NVAR * 512;
+  const int O_SEGK = O_SEGH + m * 512;
   unsigned walked = 0;
   int slot = a.head[q0 + cell];
   while (slot >= 0 && ierr == 1) {
@@ -1518,21 +1520,31 @@ __device__ __forceinline__ void adjoint_cell(const GcAdjArgs& a, unsigned q0, un
       break;
     }
     // replay forward from the checkpoint (bit-identical to the recorded run),
-    // keeping the start state of each of the segment's steps; SEGH holds the
-    // step sizes last step first
+    // keeping the start state and the stage vectors of each of the segment's
+    // steps; the checkpoint step's stage vectors come from the recording when
+    // it kept them
     const double* y0 = a.recY + (size_t)ys * NVAR;
+    const double* kk = a.recK ? a.recK + (size_t)ys * 4 * NVAR : nullptr;
     vpass<NVAR>([&](int i) { return y0[i]; }, [&](int i, double v) { b.st(O_Y + i * 512, v); });
     for (int q = 0; q < len; ++q) {
       vpass<NVAR>([&](int i) { return b.ld(O_Y + i * 512); },
                   [&](int i, double v) { b.st(O_SEG + (q * NVAR + i) * 512, v); });
-      if (q == len - 1) break;
       const double H = b.ld(O_SEGH + (len - 1 - q) * 512);
-      fun<false>(b, false, O_FCN0);
-      if (jac_lu<false>(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
-        ierr = -8;
-        break;
+      if (q == 0 && kk) {
+        if (len == 1) break;
+        vpass<4 * NVAR>([&](int k) { return kk[k]; },
+                        [&](int k, double v) { b.st(O_K + k * 512, v); });
+      } else {
+        fun<false>(b, false, O_FCN0);
+        if (jac_lu<false>(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
+          ierr = -8;
+          break;
+        }
+        stages<false>(b, true, Dir, H, ist);
+        if (q == len - 1) break;  // its factors and stages stay in place
+        vpass<4 * NVAR>([&](int k) { return b.ld(O_K + k * 512); },
+                        [&](int k, double v) { b.st(O_SEGK + (q * 4 * NVAR + k) * 512, v); });
       }
-      stages<false>(b, true, Dir, H, ist);
       vpass<NVAR>(  // Ynew and the accepted update, as the stepper
           [&](int k) {
             double yn = b.ld(O_Y + k * 512);
@@ -1542,25 +1554,25 @@ __device__ __forceinline__ void adjoint_cell(const GcAdjArgs& a, unsigned q0, un
           [&](int k, double v) { b.st(O_Y + k * 512, v); });
     }
     if (ierr != 1) break;
-    // the adjoint through the segment's steps, last first; the checkpoint
-    // step's stage vectors come from the recording when it kept them
+    // the adjoint through the segment's steps, last first: each step's start
+    // state and stage vectors back in place and its factors recomputed
+    // (the last step's are still in place from the replay)
     for (int q = len - 1; q >= 0; --q) {
       const double H = b.ld(O_SEGH + (len - 1 - q) * 512);
-      vpass<NVAR>([&](int i) { return b.ld(O_SEG + (q * NVAR + i) * 512); },
-                  [&](int i, double v) { b.st(O_Y + i * 512, v); });
-      const bool kept = q == 0 && a.recK != nullptr;
-      if (kept) {
-        const double* kk = a.recK + (size_t)ys * 4 * NVAR;
-        vpass<4 * NVAR>([&](int k) { return kk[k]; },
-                        [&](int k, double v) { b.st(O_K + k * 512, v); });
-      } else {
-        fun<false>(b, false, O_FCN0);
+      if (q != len - 1 || (q == 0 && kk)) {
+        vpass<NVAR>([&](int i) { return b.ld(O_SEG + (q * NVAR + i) * 512); },
+                    [&](int i, double v) { b.st(O_Y + i * 512, v); });
+        if (q == 0 && kk)
+          vpass<4 * NVAR>([&](int k) { return kk[k]; },
+                          [&](int k, double v) { b.st(O_K + k * 512, v); });
+        else
+          vpass<4 * NVAR>([&](int k) { return b.ld(O_SEGK + (q * 4 * NVAR + k) * 512); },
+                          [&](int k, double v) { b.st(O_K + k * 512, v); });
+        if (jac_lu<false>(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
+          ierr = -8;
+          break;
+        }
       }
-      if (jac_lu<false>(b, 1.0 / ((Dir * H) * ros_Gamma[0]))) {
-        ierr = -8;
-        break;
-      }
-      if (!kept) stages<false>(b, true, Dir, H, ist);
       adj_step(b, Dir, H, clamp);
     }
   }

@@ -324,10 +324,11 @@ int gc_amd_record_batch_device(int nCell, double Tstart, double Tend, double *C,
  * dL/dRCONST.  The record arguments are those of a successful
  * gc_amd_record_batch_device call (same pools and `every`, 1..64).  Each
  * segment of a cell's steps is replayed forward from its checkpoint (bit for
- * bit the recorded trajectory), then each step's factors and stages are
- * recomputed (a checkpoint step's stages are taken from recK when it is not
- * null: the recK of the recording) and the adjoint is swept through it with
- * transposed solves on the same factors (KppSolveTR order).  C supplies the fixed species;
+ * bit the recorded trajectory), keeping each step's stage vectors (a
+ * checkpoint step's are taken from recK when it is not null: the recK of the
+ * recording); then, last step first, each step's factors are recomputed and
+ * the adjoint is swept through it with transposed solves on the same factors
+ * (KppSolveTR order).  C supplies the fixed species;
  * ICNTRL(1) must be non-zero (autonomous, as GEOS-Chem sets it) and
  * ICNTRL(16) = 1 (clamp) is honoured.  IERR(c) = 1, -3 for a non-autonomous
  * cell (gradients 0), -8 if a recorded step's matrix is singular, -5 if the
