@@ -21,6 +21,11 @@ the sweep; measured HBM traffic already stored there under ``profiles`` is
 kept.  Usage::
 
     python geos-chem-gpu-hybrid_amd/autotune.py [--mech fullchem] [--cells N] [--passes 2]
+
+``--variants`` instead runs one solve of each representative launch shape in
+VARIANTS (after a 1 GiB elementwise copy that calibrates the HBM counters and
+the first solve), each checked bit-identical, and prints one line per shape in
+launch order: the sequence a profiler's per-dispatch records follow.
 """
 from __future__ import annotations
 
@@ -40,6 +45,21 @@ SPACE = {
     "cells_per_wave": [16, 32, 48, 64],
     "grain": [1, 16, 64, 256, 1024, 4096],
 }
+# representative points of the axes, one solve each under --variants
+VARIANTS = [
+    ("persistent, 32-entry LU segments, 8 waves/CU, 64 cells/wave (selected)", "persistent",
+     {"variant": 1, "block": 256, "blocks_per_cu": 2, "cells_per_wave": 64, "grain": 64}),
+    ("persistent, 24-entry LU segments, 12 waves/CU", "persistent",
+     {"variant": 3, "block": 256, "blocks_per_cu": 3, "cells_per_wave": 64, "grain": 64}),
+    ("persistent, 16-entry LU segments, 16 waves/CU", "persistent",
+     {"variant": 2, "block": 256, "blocks_per_cu": 4, "cells_per_wave": 64, "grain": 64}),
+    ("persistent, 32-entry segments, 32 cells/wave", "persistent",
+     {"variant": 1, "block": 256, "blocks_per_cu": 2, "cells_per_wave": 32, "grain": 64}),
+    ("persistent, 32-entry segments, 16 cells/wave", "persistent",
+     {"variant": 1, "block": 256, "blocks_per_cu": 2, "cells_per_wave": 16, "grain": 64}),
+    ("plain (one cell per lane, no refill), 32-entry segments", "plain",
+     {"variant": 1, "block": 256, "blocks_per_cu": 2, "cells_per_wave": 64, "grain": 64}),
+]
 LDS_PER_CU = 160 * 1024
 SEG = {1: 32, 2: 16, 3: 24}
 WAVES_PER_SIMD = {1: 2, 2: 4, 3: 3}
@@ -76,6 +96,8 @@ def main(argv=None):
     ap.add_argument("--cells", type=int, default=0, help="first N cells of the grid (0: all)")
     ap.add_argument("--passes", type=int, default=2)
     ap.add_argument("--out", default="")
+    ap.add_argument("--variants", action="store_true",
+                    help="one solve per shape in VARIANTS (for a profiler), no sweep")
     args = ap.parse_args(argv)
 
     import numpy as np
@@ -85,6 +107,14 @@ def main(argv=None):
 
     mech = mechanism.load(args.mech)
     lib = L.load(args.mech)
+    if args.variants:
+        # counter calibration: an elementwise copy reading and writing 1 GiB
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log("dispatch group 0: elementwise copy of 1 GiB (reads 1 GiB, writes 1 GiB)")
     d = inputs.grid_4x5_cells(mech, seed=1)
     if args.cells:
         d = inputs.subset(d, np.arange(args.cells))
@@ -112,6 +142,23 @@ def main(argv=None):
             log(f"  {t}: results differ from the first solve -- rejected")
             return None
         return out.t_kernel
+
+    if args.variants:
+        rows = []
+        for k, (label, kernel, t) in enumerate(VARIANTS, start=1):
+            L.set_tuning(lib, **t)
+            b.C.copy_(C0)
+            out = solver.solve(b, kernel=kernel, out=out)
+            torch.cuda.synchronize()
+            same = (np.array_equal(out.C.cpu().numpy().view(np.int64), ref_C)
+                    and np.array_equal(out.ISTAT.cpu().numpy(), ref_I))
+            rows.append({"label": label, "kernel": kernel, **t, "s": out.t_kernel,
+                         "bit_identical": bool(same)})
+            log(f"dispatch group {k}: {label}: {kernel} {t}: {out.t_kernel:.3f} s, "
+                f"bit-identical {same}")
+        print(json.dumps({"workload": f"{args.mech} 4x5 grid seed 1, {n} cells, cost order",
+                          "variants": rows}), flush=True)
+        return 0 if all(r["bit_identical"] for r in rows) else 1
 
     cur = dict(L.get_tuning(lib))
     cur["blocks_per_cu"] = 0

@@ -9,8 +9,13 @@
 //   stream   plain streaming read of the whole buffer (all waves, grid-stride)
 //   gather   random 512-byte wave reads, 8 in flight per wave, with one
 //            512-byte wave write per `loads_per_store` reads
-// for a given block size (doubles per lane) and waves per CU.  It prints one
-// JSON line.  Usage: gc_membench [block_doubles waves_per_cu loads_per_store]
+//   chain    latency of one 512-byte wave read whose address depends on the
+//            previous read (one wave per CU, each in its own block): the
+//            round trip a chain of dependent operations pays per link
+// for a given block size (doubles per lane) and waves per CU; the working set
+// of the gather is CUs * waves_per_cu * block_doubles * 512 bytes, so small
+// blocks measure the shape out of the on-chip caches.  It prints one JSON
+// line.  Usage: gc_membench [block_doubles waves_per_cu loads_per_store]
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
@@ -48,6 +53,24 @@ __global__ void __launch_bounds__(256) k_gather(double* buf, long block_elems, i
     }
   }
   out[wave * 64 + lane] = acc;
+}
+
+// One wave per block: `iters` dependent reads at random elements of the
+// wave's block (the buffer is zero, so the value only feeds the address).
+__global__ void __launch_bounds__(64) k_chain(const double* buf, long block_elems, int iters,
+                                             double* out) {
+  const long wave = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const double* base = buf + wave * block_elems * 64;
+  unsigned s = 12345u + (unsigned)wave * 7919u;
+  const unsigned nel = (unsigned)block_elems;
+  double v = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    s = s * 1664525u + 1013904223u;
+    const unsigned e = __builtin_amdgcn_readfirstlane(((s >> 8) + (v != 0.0 ? 1u : 0u)) % nel);
+    v = base[(long)e * 64 + lane];
+  }
+  out[wave * 64 + lane] = v;
 }
 
 __global__ void __launch_bounds__(256) k_stream(const double* buf, long n, double* out) {
@@ -111,10 +134,21 @@ int main(int argc, char** argv) {
   const double loads = (double)waves * iters * kLoads;
   const double stores = lps > 0 ? (double)waves * ((double)iters * kLoads / lps) : 0.0;
   const double gather_tbs = (loads + stores) * 512.0 / (ms * 1e-3) / 1e12;
+
+  // the chain: one wave per CU, blocks 0..cus-1 of the buffer (cus <= waves)
+  const int links = 20000;
+  hipLaunchKernelGGL(k_chain, dim3(cus), dim3(64), 0, 0, buf, block, 100, out);
+  hipEventRecord(a, 0);
+  hipLaunchKernelGGL(k_chain, dim3(cus), dim3(64), 0, 0, buf, block, links, out);
+  hipEventRecord(b, 0);
+  if (!ok(hipEventSynchronize(b), "chain")) return 1;
+  hipEventElapsedTime(&ms, a, b);
+  const double chain_ns = ms * 1e6 / links;
   printf("{\"device\": \"%s\", \"cus\": %d, \"block_doubles\": %ld, \"waves_per_cu\": %d, "
          "\"loads_per_store\": %d, \"buffer_gb\": %.2f, \"stream_read_tbs\": %.3f, "
-         "\"gather_tbs\": %.3f}\n",
-         prop.gcnArchName, cus, block, wpc, lps, n * 8.0 / 1e9, stream_tbs, gather_tbs);
+         "\"gather_tbs\": %.3f, \"chain_ns\": %.1f, \"chain_set_mb\": %.1f}\n",
+         prop.gcnArchName, cus, block, wpc, lps, n * 8.0 / 1e9, stream_tbs, gather_tbs, chain_ns,
+         cus * block * 512.0 / 1e6);
   hipFree(buf);
   hipFree(out);
   return 0;
