@@ -129,8 +129,8 @@ class Mechanism:
 
     def compulsory_bytes_per_cell(self) -> int:
         """Bytes a cell's solve must move to/from HBM: read C(NSPEC), RCONST,
-        ICNTRL(20), RCNTRL(20), Perm; write C(NVAR part is what changes, the
-        whole column is written), ISTAT(8), RSTAT(3), IERR."""
+        ICNTRL(20), RCNTRL(20), Perm; write the NVAR part of C (the fixed
+        species are not written back), ISTAT(8), RSTAT(3), IERR."""
         rd = 8 * self.nspec + 8 * self.nreact + 4 * 20 + 8 * 20 + 4
         wr = 8 * self.nvar + 4 * 8 + 8 * 3 + 4
         return rd + wr

@@ -435,6 +435,30 @@ def test_singular_retry_path(kernel):
     assert np.all(g[1][:n // 2, 7] >= 1)
 
 
+def test_singular_exit_on_gpu():
+    """Six singular factorisations in a row: IERR -8 from both kernels, equal
+    to the oracle (whose counters the CPU known-answer test fixes by hand:
+    ISTAT = [1,1,0,0,0,6,0,6]).  auto1 cells with k = 0 and Hstart = inf over
+    [0, inf): ghinv = 0, so the prepared diagonal is 0 on every try."""
+    ir = mechanism.load("auto1").ir
+    o = Oracle(ir, pow_mode="cr")
+    n = 130
+    rng = np.random.default_rng(7)
+    C = np.stack([np.exp(rng.uniform(0.0, 5.0, n)), np.ones(n)], axis=1)
+    I = np.zeros((n, 20), np.int32)
+    I[:, 0], I[:, 2] = 1, 4
+    Rc = np.zeros((n, 20))
+    Rc[:, 2] = np.inf
+    d = dict(C=C, RCONST=np.zeros((n, 1)), ICNTRL=I, RCNTRL=Rc, ATOL=np.array([1.0]),
+             RTOL=np.array([1e-3]), Perm=(rng.permutation(n) + 1).astype(np.int32),
+             Tstart=0.0, Tend=np.inf)
+    ref = _run_oracle(o, d)
+    assert np.all(ref[3] == -8) and np.all(ref[1] == np.array([1, 1, 0, 0, 0, 6, 0, 6]))
+    for kernel in ("plain", "persistent"):
+        print(f"case: singular exit, auto1, {n} cells, {kernel}", flush=True)
+        _assert_bitexact(_gpu(d, kernel, mech="auto1"), ref, 1)
+
+
 def test_step_too_small_on_gpu(mech, oracle_cr):
     """T + 0.1*H == T at T = 1e20: IERR -7 (gckpp_BatchIntegrator.F90:541-545)
     before any step with the default Hstart, and after an accepted step and

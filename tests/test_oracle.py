@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 from scipy.integrate import solve_ivp
 
-from gcchem_amd import inputs, synthetic
+from gcchem_amd import inputs, mechanism, synthetic
 from oracle import Oracle
 
 
@@ -290,6 +290,38 @@ def _quad1_case(c=QUAD1):
     Rc = np.zeros((1, 20))
     Rc[0, 2] = c["Hstart"]
     return ir, C, R, I, Rc
+
+
+def singular_exit_cells(n):
+    """Cells of the auto1 mechanism (X -> 2X, so J = +k) with k = 0 and
+    Hstart = inf over [0, inf): H = min(max(Hmin, Hstart), Hmax) = inf, so
+    ghinv = 1/(0.5 H) = 0 and the prepared diagonal -J + ghinv is 0 on every
+    try, and halving H leaves it inf."""
+    rng = np.random.default_rng(7)
+    C = np.stack([np.exp(rng.uniform(0.0, 5.0, n)), np.ones(n)], axis=1)
+    I = np.zeros((n, 20), np.int32)
+    I[:, 0], I[:, 2] = 1, 4
+    Rc = np.zeros((n, 20))
+    Rc[:, 2] = np.inf
+    return dict(C=C, RCONST=np.zeros((n, 1)), ICNTRL=I, RCNTRL=Rc, ATOL=np.array([1.0]),
+                RTOL=np.array([1e-3]), Perm=np.arange(1, n + 1, dtype=np.int32),
+                Tstart=0.0, Tend=np.inf)
+
+
+def test_singular_six_times_known_answer():
+    """Six singular factorisations in a row end a cell with IERR -8: after
+    each of the first five H is halved and the matrix prepared again, the
+    sixth returns (gckpp_BatchIntegrator.F90:572-592).  Counters by hand: one
+    Fun and one Jac_SP for the step, six decompositions, all six singular,
+    no stage solve, no step taken; C comes back unchanged."""
+    d = singular_exit_cells(4)
+    for pm in ("cr", "libm"):
+        o = Oracle(mechanism.load("auto1").ir, pow_mode=pm)
+        C, IS, RS, IE = o.integrate_batch(d["Tstart"], d["Tend"], d["C"], d["RCONST"], d["ATOL"],
+                                          d["RTOL"], d["ICNTRL"], d["RCNTRL"])[:4]
+        assert np.all(IE == -8)
+        assert np.all(IS == np.array([1, 1, 0, 0, 0, 6, 0, 6]))
+        assert np.array_equal(C[:, 0], d["C"][:, 0])
 
 
 def test_step_control_known_answer():
