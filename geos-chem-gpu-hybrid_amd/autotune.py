@@ -98,6 +98,8 @@ def main(argv=None):
     ap.add_argument("--out", default="")
     ap.add_argument("--variants", action="store_true",
                     help="one solve per shape in VARIANTS (for a profiler), no sweep")
+    ap.add_argument("--only", type=int, default=0,
+                    help="with --variants: only the shape of this 1-based index")
     args = ap.parse_args(argv)
 
     import numpy as np
@@ -146,6 +148,8 @@ def main(argv=None):
     if args.variants:
         rows = []
         for k, (label, kernel, t) in enumerate(VARIANTS, start=1):
+            if args.only and k != args.only:
+                continue
             L.set_tuning(lib, **t)
             b.C.copy_(C0)
             out = solver.solve(b, kernel=kernel, out=out)

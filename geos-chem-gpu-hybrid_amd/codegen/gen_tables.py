@@ -36,9 +36,13 @@ Tables (int32 words):
   Products: ``rate_y``/``rate_yn`` (A(r), V = Y or Ynew; gckpp_Function.F90:
   2215-3248), ``bjac`` (Jac_SP's B; gckpp_Jacobian.F90:63-3673).
 * ``pd_y``/``pd_yn``: per species 4-word header ``{nP, nD, V(i), i*512}``
-  then nP 4-word terms ``{soff, 0, coef_lo, coef_hi}`` (P_VAR(i) over A) and
-  nD 4-word product terms ``{f1, f2, f3, ci}`` (D_VAR(i), each term
-  ((lits[ci]*f1)*f2)*f3 evaluated in place, missing factors reading ONE);
+  then its 4-word P terms ``{soff, 0, coef_lo, coef_hi}`` (P_VAR(i) over A)
+  and D product terms ``{f1, f2, f3, ci}`` (D_VAR(i), each term
+  ((lits[ci]*f1)*f2)*f3 evaluated in place, missing factors reading ONE),
+  nP and nD padded to multiples of 4 with terms of value -0.0 (an exact
+  no-op: x + (-0.0) == x), interleaved in the kernel's batch order: batch j
+  holds P terms 8j..8j+7 then D terms 4j..4j+3 (fewer where the sums end),
+  so one batch's loads of both sums go out together;
   Vdot(i) = P_VAR(i) - D_VAR(i)*V(i).  (gckpp_Function.F90:3252-5058)
 * ``lu<S>``: fused Jac_SP JVS sums + matrix preparation + KppDecomp with the
   active row segment (at most S entries, plus one spare slot) in LDS; 4-word
@@ -46,8 +50,12 @@ Tables (int32 words):
   entry of the segment (flags: first/last term of an entry, diagonal, slot
   index << 8; T a multiple of 4); PIVL {2, slot_j, U(j,j),
   nupd} and PIVS {3, L(k,j), 0, nupd} + nupd/2 4-word update pairs
-  {t, U(j,c), t, U(j,c)} (nupd a multiple of 4); STORE {4, n, 0, 0} + n/2
-  {slot, LU, slot, LU} (n a multiple of 2); END {0,...}.
+  {t, U(j,c), t, U(j,c)} (nupd a multiple of 4); PIVG {7, G, 0, 0} + G
+  consecutive pivot records of at most 4 updates each, 12 words apiece
+  {2 or 3, slot_j or 0, U(j,j) or L(k,j), 0, t, U, t, U, t, U, t, U}, whose
+  data loads the kernel issues together before applying them in order;
+  STORE {4, n, 0, 0} + n/2 {slot, LU, slot, LU} (n a multiple of 2); END
+  {0,...}.
   (gckpp_Jacobian.F90:3674-20887, gckpp_BatchIntegrator.F90:573-577,
   gckpp_LinearAlgebra.F90:46-84)
 * ``fwd<s>`` (stage vector K_s) and ``tr1``/``tr2`` (the TMP vector): solve
@@ -83,6 +91,9 @@ import sys
 from pathlib import Path
 
 SEGMENTS = (16, 24, 32)  # LU row-segment sizes compiled (LDS entries per lane)
+
+
+PIVG_MAX = 3   # pivot records per PIVG group (the kernel has forms for 2 and 3)
 
 
 def mask_words(n: int) -> int:
@@ -145,16 +156,17 @@ class Tables:
         self.ir = ir
         self.E = layout(ir)
         self.lits: list[float] = []
-        self.lit_index: dict[float, int] = {}
+        self.lit_index: dict[bytes, int] = {}
 
     def off(self, arr: str, k: int) -> int:
         return (self.E[arr] + k) * 512
 
     def lit_idx(self, v: float) -> int:
-        if v not in self.lit_index:
-            self.lit_index[v] = len(self.lits)
+        key = struct.pack("<d", v)  # by bit pattern: -0.0 is its own literal
+        if key not in self.lit_index:
+            self.lit_index[key] = len(self.lits)
             self.lits.append(v)
-        return self.lit_index[v]
+        return self.lit_index[key]
 
     @staticmethod
     def lit_value(text: str) -> float:
@@ -226,14 +238,20 @@ class Tables:
         return [*data, *([one] * (3 - len(data))), self.lit_idx(c)]
 
     def pd(self, vbase) -> list[int]:
+        one = self.off("ONE", 0)
+        ppad = [one, 0, *_coef_words(-0.0)]              # -0.0 * 1.0
+        dpad = [one, one, one, self.lit_idx(-0.0)]       # ((-0.0 * 1) * 1) * 1
         out = []
         for i in range(self.ir["nvar"]):
-            P, D = self.ir["prod"][i], self.ir["dest"][i]
+            P = [self.sum_term(s, p, lambda x: self.data_off(x, vbase))
+                 for s, p in self.ir["prod"][i]]
+            D = [self.d_term(s, p, vbase) for s, p in self.ir["dest"][i]]
+            P += [ppad] * (-len(P) % 4)
+            D += [dpad] * (-len(D) % 4)
             out += [len(P), len(D), self.off(vbase, i), i * 512]
-            for s, p in P:
-                out += self.sum_term(s, p, lambda x: self.data_off(x, vbase))
-            for s, p in D:
-                out += self.d_term(s, p, vbase)
+            for j in range(max(-(-len(P) // 8), len(D) // 4)):
+                for w in P[8 * j:8 * j + 8] + D[4 * j:4 * j + 4]:
+                    out += w
         return out
 
     # -- Jacobian + LU ---------------------------------------------------------
@@ -282,6 +300,29 @@ class Tables:
                 w += [a, b]
             return len(ps), w
 
+        pad = (spare, self.off("ZERO", 0))
+
+        def emit(recs):
+            # runs of pivot records with at most 4 updates become PIVG groups
+            # of up to PIVG_MAX (a record alone stays PIVL/PIVS)
+            out, q = [], 0
+            while q < len(recs):
+                g = 0
+                while q + g < len(recs) and g < PIVG_MAX and len(recs[q + g][3]) <= 4:
+                    g += 1
+                if g >= 2:
+                    out += [7, g, 0, 0]
+                    for kind, f1, opnd, ups in recs[q:q + g]:
+                        full = list(ups) + [pad] * (4 - len(ups))  # exactly 4 updates
+                        out += [kind, f1, opnd, 0] + pairs(full, 4, pad)[1]
+                    q += g
+                    continue
+                kind, f1, opnd, ups = recs[q]
+                n, w = pairs(ups, 4, pad)
+                out += ([2, f1, opnd, n] if kind == 2 else [3, opnd, 0, n]) + w
+                q += 1
+            return out
+
         for k in range(nvar):
             cols = icol[crow[k]:crow[k + 1]]
             pos = {c: crow[k] + n for n, c in enumerate(cols)}
@@ -291,6 +332,7 @@ class Tables:
                 lo = segcols[0]
                 ops += self.entry_group([(slot[c], c == k, jvs[pos[c]][2], pos[c])
                                          for c in segcols], spare)
+                recs = []
                 for kk in range(crow[k], diag[k]):
                     j = icol[kk]
                     if j > segcols[-1]:
@@ -298,12 +340,11 @@ class Tables:
                     ups = [(slot[icol[jj]], self.off("LU", jj))
                            for jj in range(diag[j] + 1, crow[j + 1]) if icol[jj] in slot]
                     if j in slot:
-                        n, w = pairs(ups, 4, (spare, self.off("ZERO", 0)))
-                        ops += [2, slot[j], self.off("LU", diag[j]), n] + w
+                        recs.append((2, slot[j], self.off("LU", diag[j]), ups))
                     elif ups:
                         assert j < lo
-                        n, w = pairs(ups, 4, (spare, self.off("ZERO", 0)))
-                        ops += [3, self.off("LU", kk), 0, n] + w
+                        recs.append((3, 0, self.off("LU", kk), ups))
+                ops += emit(recs)
                 st = [(slot[c], self.off("LU", pos[c])) for c in segcols]
                 n, w = pairs(st, 2, (spare, self.off("SINK", 0)))
                 ops += [4, n, 0, 0] + w

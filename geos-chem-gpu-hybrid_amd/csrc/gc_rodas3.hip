@@ -249,68 +249,44 @@ __device__ void run_products(const Blk& b, const int* table, const int* grp_) {
   prod_group<4, true>(b, rec, grp[7]);
 }
 
-// Sum of n >= 1 terms {soff, 0, coef}: acc = c0*x0, then acc = acc + c*x in
-// order (signs are folded into the coefficients).  The first term's words
-// come in `first` (fetched with the record header); t points at the second
-// term and is advanced past the last one.
-__device__ __forceinline__ double sum_terms(const Blk& b, i4 first, ctab_t*& t, int n) {
-  double acc = dbl(first.z, first.w) * b.ld(first.x);
-  int k = 1;
-  for (; k + 8 <= n; k += 8, t += 32) {
-    i4 q[8];
+// One batch of a species' sums: PC P terms {soff, 0, coef} and DC D terms
+// {f1, f2, f3, ci} at t, all their loads issued before either sum is
+// advanced; each sum goes on in term order, its first term (fp / fd) setting
+// the accumulator.
+template <int PC, int DC>
+__device__ __forceinline__ void pd_batch(const Blk& b, ctab_t* t, bool fp, bool fd, double& P,
+                                         double& D) {
+  i4 qp[PC > 0 ? PC : 1], qd[DC > 0 ? DC : 1];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) q[r] = ld4(t + 4 * r);
-    double x[8];
+  for (int r = 0; r < PC; ++r) qp[r] = ld4(t + 4 * r);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) x[r] = b.ld(q[r].x);
+  for (int r = 0; r < DC; ++r) qd[r] = ld4(t + 4 * (PC + r));
+  double x[PC > 0 ? PC : 1], f[DC > 0 ? 3 * DC : 1];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) acc = acc + dbl(q[r].z, q[r].w) * x[r];
+  for (int r = 0; r < PC; ++r) x[r] = b.ld(qp[r].x);
+#pragma unroll
+  for (int r = 0; r < DC; ++r) {
+    f[3 * r] = b.ld(qd[r].x);
+    f[3 * r + 1] = b.ld(qd[r].y);
+    f[3 * r + 2] = b.ld(qd[r].z);
   }
-  for (; k + 4 <= n; k += 4, t += 16) {
-    const i4 q0 = ld4(t), q1 = ld4(t + 4), q2 = ld4(t + 8), q3 = ld4(t + 12);
-    const double x0 = b.ld(q0.x), x1 = b.ld(q1.x), x2 = b.ld(q2.x), x3 = b.ld(q3.x);
-    acc = acc + dbl(q0.z, q0.w) * x0;
-    acc = acc + dbl(q1.z, q1.w) * x1;
-    acc = acc + dbl(q2.z, q2.w) * x2;
-    acc = acc + dbl(q3.z, q3.w) * x3;
+#pragma unroll
+  for (int r = 0; r < PC; ++r) {
+    const double v = dbl(qp[r].z, qp[r].w) * x[r];
+    P = (r == 0 && fp) ? v : P + v;
   }
-  for (; k < n; ++k, t += 4) {
-    const i4 q = ld4(t);
-    acc = acc + dbl(q.z, q.w) * b.ld(q.x);
+#pragma unroll
+  for (int r = 0; r < DC; ++r) {
+    const double v = ((((clit_t*)gc_lits)[qd[r].w] * f[3 * r]) * f[3 * r + 1]) * f[3 * r + 2];
+    D = (r == 0 && fd) ? v : D + v;
   }
-  return acc;
-}
-
-// One D_VAR term {f1, f2, f3, ci}: ((lits[ci]*f1)*f2)*f3, the signed literal
-// first, missing factors reading ONE (1.0).
-__device__ __forceinline__ double dterm(const Blk& b, i4 q) {
-  const double c = ((clit_t*)gc_lits)[q.w];
-  return ((c * b.ld(q.x)) * b.ld(q.y)) * b.ld(q.z);
-}
-
-// Sum of n >= 1 D_VAR terms in order (the first one sets the accumulator).
-__device__ __forceinline__ double sum_dterms(const Blk& b, ctab_t*& t, int n) {
-  double acc = dterm(b, ld4(t));
-  t += 4;
-  int k = 1;
-  for (; k + 4 <= n; k += 4, t += 16) {
-    const i4 q0 = ld4(t), q1 = ld4(t + 4), q2 = ld4(t + 8), q3 = ld4(t + 12);
-    const double f00 = b.ld(q0.x), f01 = b.ld(q0.y), f02 = b.ld(q0.z);
-    const double f10 = b.ld(q1.x), f11 = b.ld(q1.y), f12 = b.ld(q1.z);
-    const double f20 = b.ld(q2.x), f21 = b.ld(q2.y), f22 = b.ld(q2.z);
-    const double f30 = b.ld(q3.x), f31 = b.ld(q3.y), f32 = b.ld(q3.z);
-    acc = acc + ((((clit_t*)gc_lits)[q0.w] * f00) * f01) * f02;
-    acc = acc + ((((clit_t*)gc_lits)[q1.w] * f10) * f11) * f12;
-    acc = acc + ((((clit_t*)gc_lits)[q2.w] * f20) * f21) * f22;
-    acc = acc + ((((clit_t*)gc_lits)[q3.w] * f30) * f31) * f32;
-  }
-  for (; k < n; ++k, t += 4) acc = acc + dterm(b, ld4(t));
-  return acc;
 }
 
 // Fun_SPLIT (gckpp_Function.F90:2170-5060): rates, then per species P_VAR,
 // D_VAR (term products evaluated in place) and Vdot = P_VAR - D_VAR*V,
-// stored at out + i*512.
+// stored at out + i*512.  A species' terms come in batches of up to 8 P and
+// 4 D terms (both sums padded to multiples of 4 with -0.0 terms), one round
+// trip per batch.
 // Masked (M): IF (DO_FUN(i)) guards both sums of species i
 // (gckpp_Function.F90:3280-5056); a species whose flag is off keeps the P_VAR
 // and D_VAR the cell's workspace holds, zero from the cell's start, so its
@@ -322,21 +298,33 @@ __device__ void fun(const Blk& b, bool ynew, int out) {
   ctab_t* t = tab(ynew ? gc_pd_yn : gc_pd_y);
   MaskWord mf;
   for (int i = 0; i < NVAR; ++i) {
-    const i4 h = ld4(t), f = ld4(t + 4);  // header and the first term
+    const i4 h = ld4(t);
     const int nP = h.x, nD = h.y, vo = h.z, oo = h.w;
-    t += 8;
-    double P = 0.0;
-    if (nP > 0) {
-      P = sum_terms(b, f, t, nP);
-    } else {
-      t -= 4;
+    t += 4;
+    const double v = b.ld(vo);
+    double P = 0.0, D = 0.0;
+    for (int kp = 0, kd = 0; kp < nP || kd < nD;) {
+      const int pc = (nP - kp < 8) ? nP - kp : 8;  // 0, 4 or 8
+      const int dc = (nD - kd < 4) ? nD - kd : 4;  // 0 or 4
+      const bool fp = kp == 0, fd = kd == 0;
+      if (pc == 8) {
+        if (dc == 4) pd_batch<8, 4>(b, t, fp, fd, P, D);
+        else pd_batch<8, 0>(b, t, fp, fd, P, D);
+      } else if (pc == 4) {
+        if (dc == 4) pd_batch<4, 4>(b, t, fp, fd, P, D);
+        else pd_batch<4, 0>(b, t, fp, fd, P, D);
+      } else {
+        pd_batch<0, 4>(b, t, fp, fd, P, D);
+      }
+      t += 4 * (pc + dc);
+      kp += pc;
+      kd += dc;
     }
-    double D = nD > 0 ? sum_dterms(b, t, nD) : 0.0;
     if (M && !mf.on(b, MECH_E_MFUN, oo >> 9)) {
       P = 0.0;
       D = 0.0;
     }
-    b.st(out + oo, P - D * b.ld(vo));
+    b.st(out + oo, P - D * v);
   }
 }
 
@@ -362,6 +350,49 @@ __device__ __forceinline__ void entry_term(const Blk& b, i4 t, double x, double 
     }
     b.lds_st(((fl >> 8) & 63) * 512, g);
   }
+}
+
+// G consecutive pivot records of at most four updates each (a PIVG group):
+// every record's data loads (its pivot operand and its four U values) are
+// issued together, then the records are applied in order, each one's
+// multiplier read from the LDS row segment after the previous one's updates.
+// Sub-record words {kind, slot_j, operand, 0, t, U, t, U, t, U, t, U}: kind 2
+// a = -W(j)/U(j,j), W(j) = -a (PIVL); kind 3 a = -L(k,j) (PIVS).
+template <int G>
+__device__ __forceinline__ ctab_t* piv_group(const Blk& b, ctab_t* op) {
+  i4 h[G], p01[G], p23[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    h[g] = ld4(op + 12 * g);
+    p01[g] = ld4(op + 12 * g + 4);
+    p23[g] = ld4(op + 12 * g + 8);
+  }
+  double v[G], u[G][4];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    v[g] = b.ld(h[g].z);
+    u[g][0] = b.ld(p01[g].y);
+    u[g][1] = b.ld(p01[g].w);
+    u[g][2] = b.ld(p23[g].y);
+    u[g][3] = b.ld(p23[g].w);
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    double a;
+    if (h[g].x == 2) {
+      a = -b.lds_ld(h[g].y) / v[g];
+      b.lds_st(h[g].y, -a);
+    } else {
+      a = -v[g];
+    }
+    const double w0 = b.lds_ld(p01[g].x), w1 = b.lds_ld(p01[g].z);
+    const double w2 = b.lds_ld(p23[g].x), w3 = b.lds_ld(p23[g].z);
+    b.lds_st(p01[g].x, w0 + a * u[g][0]);
+    b.lds_st(p01[g].z, w1 + a * u[g][1]);
+    b.lds_st(p23[g].x, w2 + a * u[g][2]);
+    b.lds_st(p23[g].z, w3 + a * u[g][3]);
+  }
+  return op + 12 * G;
 }
 
 // Jac_SP fused with Ghimj = -Jac0, diag += ghinv and KppDecomp, row segments
@@ -404,6 +435,8 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) entry_term<M>(b, tt[r], x[r], ghinv, acc, sing, mj);
       }
+    } else if (code == 7) {
+      op = (h.y == 3) ? piv_group<3>(b, op + 4) : piv_group<2>(b, op + 4);
     } else if (code == 2 || code == 3) {
       // n updates {slot, U} (n a multiple of 4; padding entries update the
       // spare slot, real ones have distinct slots): the first four are loaded

@@ -110,12 +110,21 @@ class TableMachine:
         w, p = self.pd[sfx], 0
         for _ in range(self.ir["nvar"]):
             nP, nD, vo, oo = w[p:p + 4]
+            assert nP % 4 == 0 and nD % 4 == 0
             p += 4
             P = D = 0.0
-            if nP:
-                P, p = self.sum_terms(w, p, nP)
-            if nD:
-                D, p = self.sum_dterms(w, p, nD)
+            kp = kd = 0
+            while kp < nP or kd < nD:  # batches: up to 8 P terms, then up to 4 D terms
+                pc, dc = min(8, nP - kp), min(4, nD - kd)
+                for r in range(pc):
+                    v = _coef(w[p + 2], w[p + 3]) * self.ld(w[p])
+                    P = v if kp + r == 0 else P + v
+                    p += 4
+                for r in range(dc):
+                    v = self.dterm(w, p)
+                    D = v if kd + r == 0 else D + v
+                    p += 4
+                kp, kd = kp + pc, kd + dc
             if self.do_fun is not None:
                 on = self.do_fun[:, oo // 512]
                 P, D = np.where(on, P, 0.0), np.where(on, D, 0.0)
@@ -160,6 +169,23 @@ class TableMachine:
                             sing |= np.abs(g) < 2.2250738585072014e-308
                         self.lds[(fl >> 8) & 63] = g
                     p += 4
+            elif code == 7:
+                G = w[p + 1]
+                assert 2 <= G <= 3
+                p += 4
+                for _ in range(G):
+                    kind, sj, opnd = w[p], w[p + 1], w[p + 2]
+                    if kind == 2:
+                        a = -lds(sj) / self.ld(opnd)
+                        self.lds[sj // 512] = -a
+                    else:
+                        assert kind == 3
+                        a = -self.ld(opnd)
+                    p += 4
+                    for _ in range(4):
+                        t, u = w[p], w[p + 1]
+                        self.lds[t // 512] = lds(t) + a * self.ld(u)
+                        p += 2
             elif code in (2, 3):
                 if code == 2:
                     sj = w[p + 1]
