@@ -696,12 +696,13 @@ class Tables:
 
 def _arr(name, vals, ctype="int") -> str:
     """A table as a 64-byte aligned device array.  Int tables are padded with
-    zeros to a multiple of 16 words plus 16, so the kernels may fetch 16-byte
-    groups of words (every record starts on a 4-word boundary) and read one
-    group past a record without leaving the array."""
+    zeros to a multiple of 16 words plus 32, so the kernels may fetch 16-byte
+    groups of words (every record starts on a 4-word boundary) and read up to
+    32 words past the last record (the next record's words fetched ahead of
+    time) without leaving the array."""
     vals = list(vals)
     if ctype == "int":
-        vals += [0] * ((-len(vals)) % 16 + 16)
+        vals += [0] * ((-len(vals)) % 16 + 32)
     body = ",".join(str(v) for v in vals) if vals else "0"
     return f"__device__ const {ctype} {name}[] __attribute__((aligned(64))) = {{{body}}};\n"
 

@@ -155,12 +155,15 @@ struct MaskWord {
   }
 };
 
-// Element passes: val(k) loads (and combines) what element k needs, put(k, v)
-// stores it.  VB consecutive elements are evaluated, their loads issued
-// together, before any of them is stored, so a pass keeps VB elements' loads
+// Element passes: val(k) loads what element k needs (loads only), put(k, v)
+// combines and stores it.  VB consecutive elements' loads are issued together
+// before any of them is combined or stored, so a pass keeps VB elements' loads
 // in flight instead of one round trip per element (the stores of a pass would
-// otherwise order every later load behind them).  Valid for passes in which
-// element k reads nothing that the pass stores at another index.
+// otherwise order every later load behind them).  The scheduling fence keeps
+// the compiler from sinking later elements' loads below earlier elements'
+// arithmetic, which would put a wait for memory between every two elements.
+// Valid for passes in which element k reads nothing that the pass stores at
+// another index.
 constexpr int VB = 8;
 
 template <int N, class V, class P>
@@ -170,6 +173,7 @@ __device__ __forceinline__ void vpass(V val, P put) {
     decltype(val(0)) v[VB];
 #pragma unroll
     for (int u = 0; u < VB; ++u) v[u] = val(k + u);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < VB; ++u) put(k + u, v[u]);
   }
@@ -178,7 +182,7 @@ __device__ __forceinline__ void vpass(V val, P put) {
 
 // Table words arrive in 16-byte groups (one s_load_dwordx4 each): every
 // record starts on a 4-word boundary of a 64-byte aligned table, and every
-// table has 16 words of zero padding after its last record.
+// table has at least 32 words of zero padding after its last record.
 typedef int i4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(4))) const i4 ci4_t;
 
@@ -190,45 +194,56 @@ __device__ __forceinline__ double dbl(int lo, int hi) {
 }
 
 
-// One product record {dst, lit, f0, f1, f2, f3, 0, 0} of shape (NF, LIT):
-// lit*f0*f1*... left to right.
+// R product records {dst, lit, f0, f1, f2, f3, 0, 0} of shape (NF, LIT), each
+// lit*f0*f1*... left to right: every record's factors (and literal) are
+// fetched before any product is formed -- the fence stops the compiler from
+// interleaving one record's loads with the previous record's multiplies, which
+// would cost a memory round trip per record -- then the products are stored.
 // Record words a = {dst, lit, f0, f1}, c = {f2, f3, 0, 0}.
-template <int NF, bool LIT>
-__device__ __forceinline__ double prod_rec(const Blk& b, i4 a, i4 c) {
-  double v = b.ld(a.z);
-  if (LIT) v = ((clit_t*)gc_lits)[a.y] * v;
-  if (NF > 1) v = v * b.ld(a.w);
-  if (NF > 2) v = v * b.ld(c.x);
-  if (NF > 3) v = v * b.ld(c.y);
-  return v;
+template <int NF, bool LIT, int R>
+__device__ __forceinline__ void prod_batch(const Blk& b, ctab_t* rec) {
+  i4 a[R], c[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    a[q] = ld4(rec + 8 * q);
+    if (NF > 2) c[q] = ld4(rec + 8 * q + 4);
+  }
+  double f[R][NF], lit[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (LIT) lit[q] = ((clit_t*)gc_lits)[a[q].y];
+    f[q][0] = b.ld(a[q].z);
+    if (NF > 1) f[q][1] = b.ld(a[q].w);
+    if (NF > 2) f[q][2] = b.ld(c[q].x);
+    if (NF > 3) f[q][3] = b.ld(c[q].y);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    double v = f[q][0];
+    if (LIT) v = lit[q] * v;
+#pragma unroll
+    for (int m = 1; m < NF; ++m) v = v * f[q][m];
+    b.st(a[q].x, v);
+  }
 }
 
 template <int NF, bool LIT>
 __device__ __forceinline__ ctab_t* prod_group(const Blk& b, ctab_t* rec, int n) {
   int r = 0;
-  for (; r + 4 <= n; r += 4, rec += 32) {
-    i4 a[4], c[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      a[q] = ld4(rec + 8 * q);
-      c[q] = ld4(rec + 8 * q + 4);
-    }
-    double v[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = prod_rec<NF, LIT>(b, a[q], c[q]);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) b.st(a[q].x, v[q]);
+  for (; r + 8 <= n; r += 8, rec += 64) prod_batch<NF, LIT, 8>(b, rec);
+  if (r + 4 <= n) {
+    prod_batch<NF, LIT, 4>(b, rec);
+    r += 4;
+    rec += 32;
   }
-  for (; r + 2 <= n; r += 2, rec += 16) {
-    const i4 a0 = ld4(rec), c0 = ld4(rec + 4), a1 = ld4(rec + 8), c1 = ld4(rec + 12);
-    const double v0 = prod_rec<NF, LIT>(b, a0, c0);
-    const double v1 = prod_rec<NF, LIT>(b, a1, c1);
-    b.st(a0.x, v0);
-    b.st(a1.x, v1);
+  if (r + 2 <= n) {
+    prod_batch<NF, LIT, 2>(b, rec);
+    r += 2;
+    rec += 16;
   }
   if (r < n) {
-    const i4 a0 = ld4(rec), c0 = ld4(rec + 4);
-    b.st(a0.x, prod_rec<NF, LIT>(b, a0, c0));
+    prod_batch<NF, LIT, 1>(b, rec);
     rec += 8;
   }
   return rec;
@@ -545,25 +560,42 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
 // x_i = x_i - LU*x_j ... in order, then / diag for the backward rows.
 // Masked (M): IF (DO_SLV(i)) guards the row (gckpp_LinearAlgebra.F90:
 // 653-2309), x_i stays as it was; the rows' x_i are elements xbase + i.
+// A row's header and first eight pairs arrive with the previous row (fetched
+// while that row's data loads are in flight), so a row of up to eight pairs
+// costs one memory round trip and no wait for its table words.
 template <bool DIV, bool M = false>
 __device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows, int xbase = 0) {
   MaskWord ms;
+  i4 h = ld4(t), q0 = ld4(t + 4), q1 = ld4(t + 8), q2 = ld4(t + 12), q3 = ld4(t + 16);
   for (int r = 0; r < rows; ++r) {
-    const i4 h = ld4(t), p01 = ld4(t + 4), p23 = ld4(t + 8);  // header, first four pairs
     const int n = h.x, xi = h.y, dg = h.z;
-    t += 4;
+    ctab_t* const tn = t + 4 + 2 * n;  // the next row
     double x = b.ld(xi);
-    const double x0 = x;
-    if (n > 0) {
-      const double l0 = b.ld(p01.x), x0 = b.ld(p01.y), l1 = b.ld(p01.z), x1 = b.ld(p01.w);
-      const double l2 = b.ld(p23.x), x2 = b.ld(p23.y), l3 = b.ld(p23.z), x3 = b.ld(p23.w);
-      x = x - l0 * x0;
-      x = x - l1 * x1;
-      x = x - l2 * x2;
-      x = x - l3 * x3;
-      t += 8;
+    double dv = 1.0;
+    if (DIV) dv = b.ld(dg);
+    double l[8], y[8];
+    if (n >= 4) {
+      l[0] = b.ld(q0.x), y[0] = b.ld(q0.y), l[1] = b.ld(q0.z), y[1] = b.ld(q0.w);
+      l[2] = b.ld(q1.x), y[2] = b.ld(q1.y), l[3] = b.ld(q1.z), y[3] = b.ld(q1.w);
     }
-    int u = 4;
+    if (n >= 8) {
+      l[4] = b.ld(q2.x), y[4] = b.ld(q2.y), l[5] = b.ld(q2.z), y[5] = b.ld(q2.w);
+      l[6] = b.ld(q3.x), y[6] = b.ld(q3.y), l[7] = b.ld(q3.z), y[7] = b.ld(q3.w);
+    }
+    const i4 hn = ld4(tn), n0 = ld4(tn + 4), n1 = ld4(tn + 8), n2 = ld4(tn + 12),
+             n3 = ld4(tn + 16);
+    __builtin_amdgcn_sched_barrier(0);
+    const double x0 = x;
+    if (n >= 4) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) x = x - l[k] * y[k];
+    }
+    if (n >= 8) {
+#pragma unroll
+      for (int k = 4; k < 8; ++k) x = x - l[k] * y[k];
+    }
+    t += 20;
+    int u = 8;
     for (; u + 8 <= n; u += 8, t += 16) {
       i4 q[4];
 #pragma unroll
@@ -581,16 +613,22 @@ __device__ __forceinline__ void solve_rows(const Blk& b, ctab_t* t, int rows, in
     }
     for (; u < n; u += 4, t += 8) {
       const i4 q01 = ld4(t), q23 = ld4(t + 4);
-      const double l0 = b.ld(q01.x), x0 = b.ld(q01.y), l1 = b.ld(q01.z), x1 = b.ld(q01.w);
-      const double l2 = b.ld(q23.x), x2 = b.ld(q23.y), l3 = b.ld(q23.z), x3 = b.ld(q23.w);
-      x = x - l0 * x0;
-      x = x - l1 * x1;
-      x = x - l2 * x2;
-      x = x - l3 * x3;
+      const double l0 = b.ld(q01.x), y0 = b.ld(q01.y), l1 = b.ld(q01.z), y1 = b.ld(q01.w);
+      const double l2 = b.ld(q23.x), y2 = b.ld(q23.y), l3 = b.ld(q23.z), y3 = b.ld(q23.w);
+      x = x - l0 * y0;
+      x = x - l1 * y1;
+      x = x - l2 * y2;
+      x = x - l3 * y3;
     }
-    if (DIV) x = x / b.ld(dg);
+    if (DIV) x = x / dv;
     if (M && !ms.on(b, MECH_E_MSLV, (xi >> 9) - xbase)) x = x0;
     b.st(xi, x);
+    h = hn;
+    q0 = n0;
+    q1 = n1;
+    q2 = n2;
+    q3 = n3;
+    t = tn;
   }
 }
 
@@ -605,67 +643,107 @@ template <bool M>
 __device__ __forceinline__ void solve_bwd_lds(const Blk& b, ctab_t* t, int rows, int xbase) {
   b.lds_st(GC_SEG * 512, 0.0);
   MaskWord ms;
+  // a row's header and the words of its first eight pairs arrive with the
+  // previous row, as in solve_rows
+  i4 h = ld4(t), q0 = ld4(t + 4), q1 = ld4(t + 8), q2 = ld4(t + 12), q3 = ld4(t + 16);
   for (int r = 0; r < rows; ++r) {
-    const i4 h = ld4(t);
-    t += 4;
     const int nc = h.x & 0xffff, nh = h.x >> 16;
+    ctab_t* const tn = t + 4 + 2 * (nc + nh);  // the next row
     double x = b.ld(h.y);
+    const double dv = b.ld(h.z);
+    // the first eight pairs: cold ones first (two loads), then LDS ones
+    // (their x_j from the slot); cc cold and hc LDS pairs among them
+    const int cc = nc < 8 ? nc : 8;
+    const int hc = nh < 8 - cc ? nh : 8 - cc;
+    double l[8], y[8];
+    if (cc >= 4) {
+      l[0] = b.ld(q0.x), y[0] = b.ld(q0.y), l[1] = b.ld(q0.z), y[1] = b.ld(q0.w);
+      l[2] = b.ld(q1.x), y[2] = b.ld(q1.y), l[3] = b.ld(q1.z), y[3] = b.ld(q1.w);
+    } else if (hc >= 4) {
+      l[0] = b.ld(q0.x), y[0] = b.lds_ld(q0.y), l[1] = b.ld(q0.z), y[1] = b.lds_ld(q0.w);
+      l[2] = b.ld(q1.x), y[2] = b.lds_ld(q1.y), l[3] = b.ld(q1.z), y[3] = b.lds_ld(q1.w);
+    }
+    if (cc == 8) {
+      l[4] = b.ld(q2.x), y[4] = b.ld(q2.y), l[5] = b.ld(q2.z), y[5] = b.ld(q2.w);
+      l[6] = b.ld(q3.x), y[6] = b.ld(q3.y), l[7] = b.ld(q3.z), y[7] = b.ld(q3.w);
+    } else if (cc + hc == 8) {
+      l[4] = b.ld(q2.x), y[4] = b.lds_ld(q2.y), l[5] = b.ld(q2.z), y[5] = b.lds_ld(q2.w);
+      l[6] = b.ld(q3.x), y[6] = b.lds_ld(q3.y), l[7] = b.ld(q3.z), y[7] = b.lds_ld(q3.w);
+    }
+    const i4 hn = ld4(tn), n0 = ld4(tn + 4), n1 = ld4(tn + 8), n2 = ld4(tn + 12),
+             n3 = ld4(tn + 16);
+    __builtin_amdgcn_sched_barrier(0);
     const double x0 = x;
-    int u = 0;
+    if (cc + hc >= 4) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) x = x - l[k] * y[k];
+    }
+    if (cc + hc == 8) {
+#pragma unroll
+      for (int k = 4; k < 8; ++k) x = x - l[k] * y[k];
+    }
+    t += 20;  // the pairs after the first eight
+    int u = cc;
     for (; u + 8 <= nc; u += 8, t += 16) {
       i4 q[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) q[k] = ld4(t + 4 * k);
-      double l[8], y[8];
+      double lq[8], yq[8];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        l[2 * k] = b.ld(q[k].x);
-        y[2 * k] = b.ld(q[k].y);
-        l[2 * k + 1] = b.ld(q[k].z);
-        y[2 * k + 1] = b.ld(q[k].w);
+        lq[2 * k] = b.ld(q[k].x);
+        yq[2 * k] = b.ld(q[k].y);
+        lq[2 * k + 1] = b.ld(q[k].z);
+        yq[2 * k + 1] = b.ld(q[k].w);
       }
 #pragma unroll
-      for (int k = 0; k < 8; ++k) x = x - l[k] * y[k];
+      for (int k = 0; k < 8; ++k) x = x - lq[k] * yq[k];
     }
     for (; u < nc; u += 4, t += 8) {
       const i4 q01 = ld4(t), q23 = ld4(t + 4);
-      const double l0 = b.ld(q01.x), x0 = b.ld(q01.y), l1 = b.ld(q01.z), x1 = b.ld(q01.w);
-      const double l2 = b.ld(q23.x), x2 = b.ld(q23.y), l3 = b.ld(q23.z), x3 = b.ld(q23.w);
-      x = x - l0 * x0;
-      x = x - l1 * x1;
-      x = x - l2 * x2;
-      x = x - l3 * x3;
+      const double l0 = b.ld(q01.x), y0 = b.ld(q01.y), l1 = b.ld(q01.z), y1 = b.ld(q01.w);
+      const double l2 = b.ld(q23.x), y2 = b.ld(q23.y), l3 = b.ld(q23.z), y3 = b.ld(q23.w);
+      x = x - l0 * y0;
+      x = x - l1 * y1;
+      x = x - l2 * y2;
+      x = x - l3 * y3;
     }
-    u = 0;
+    u = hc;
     for (; u + 8 <= nh; u += 8, t += 16) {
       i4 q[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) q[k] = ld4(t + 4 * k);
-      double l[8], y[8];
+      double lq[8], yq[8];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        l[2 * k] = b.ld(q[k].x);
-        l[2 * k + 1] = b.ld(q[k].z);
-        y[2 * k] = b.lds_ld(q[k].y);
-        y[2 * k + 1] = b.lds_ld(q[k].w);
+        lq[2 * k] = b.ld(q[k].x);
+        lq[2 * k + 1] = b.ld(q[k].z);
+        yq[2 * k] = b.lds_ld(q[k].y);
+        yq[2 * k + 1] = b.lds_ld(q[k].w);
       }
 #pragma unroll
-      for (int k = 0; k < 8; ++k) x = x - l[k] * y[k];
+      for (int k = 0; k < 8; ++k) x = x - lq[k] * yq[k];
     }
     for (; u < nh; u += 4, t += 8) {
       const i4 q01 = ld4(t), q23 = ld4(t + 4);
       const double l0 = b.ld(q01.x), l1 = b.ld(q01.z), l2 = b.ld(q23.x), l3 = b.ld(q23.z);
-      const double x0 = b.lds_ld(q01.y), x1 = b.lds_ld(q01.w);
-      const double x2 = b.lds_ld(q23.y), x3 = b.lds_ld(q23.w);
-      x = x - l0 * x0;
-      x = x - l1 * x1;
-      x = x - l2 * x2;
-      x = x - l3 * x3;
+      const double y0 = b.lds_ld(q01.y), y1 = b.lds_ld(q01.w);
+      const double y2 = b.lds_ld(q23.y), y3 = b.lds_ld(q23.w);
+      x = x - l0 * y0;
+      x = x - l1 * y1;
+      x = x - l2 * y2;
+      x = x - l3 * y3;
     }
-    x = x / b.ld(h.z);
+    x = x / dv;
     if (M && !ms.on(b, MECH_E_MSLV, (h.y >> 9) - xbase)) x = x0;
     b.st(h.y, x);
     if (h.w >= 0) b.lds_st(h.w, x);
+    h = hn;
+    q0 = n0;
+    q1 = n1;
+    q2 = n2;
+    q3 = n3;
+    t = tn;
   }
 }
 
@@ -760,57 +838,98 @@ __device__ void step_functions(const Blk& b, const Opts& o, double T, int* ist) 
     const double Delta = sqrt(ROUNDOFF) * dmax(1.0e-6, fabs(T));
     fun<M>(b, false, O_DFDT);
     ist[0] += 1;
-    vpass<NVAR>(
-        [&](int k) { return (b.ld(O_DFDT + k * 512) - b.ld(O_FCN0 + k * 512)) / Delta; },
-        [&](int k, double v) { b.st(O_DFDT + k * 512, v); });
+    struct L2 {
+      double d, f;
+    };
+    vpass<NVAR>([&](int k) { return L2{b.ld(O_DFDT + k * 512), b.ld(O_FCN0 + k * 512)}; },
+                [&](int k, const L2& l) { b.st(O_DFDT + k * 512, (l.d - l.f) / Delta); });
   }
 }
 
-// Ynew = Y + sum_j A_sj K_j, the point of stage s >= 2 (ros_NewF).
-__device__ void stage_point(const Blk& b, int s) {
+// Ynew = Y + sum_j A_sj K_j, the point of stage S >= 2 (ros_NewF).
+template <int S>
+__device__ void stage_point(const Blk& b) {
+  struct L {
+    double y, k[S];
+  };
   vpass<NVAR>(
       [&](int k) {
-        double y = b.ld(O_Y + k * 512);
-        for (int j = 0; j < s; ++j) y = y + ros_A[s * (s - 1) / 2 + j] * b.ld(O_K + j * O_VEC + k * 512);
-        return y;
+        L l;
+        l.y = b.ld(O_Y + k * 512);
+#pragma unroll
+        for (int j = 0; j < S; ++j) l.k[j] = b.ld(O_K + j * O_VEC + k * 512);
+        return l;
       },
-      [&](int k, double y) { b.st(O_YNEW + k * 512, y); });
+      [&](int k, const L& l) {
+        double y = l.y;
+#pragma unroll
+        for (int j = 0; j < S; ++j) y = y + ros_A[S * (S - 1) / 2 + j] * l.k[j];
+        b.st(O_YNEW + k * 512, y);
+      });
+}
+
+__device__ __forceinline__ void stage_point(const Blk& b, int s) {
+  if (s == 2) stage_point<2>(b);
+  else stage_point<3>(b);
 }
 
 // The four stages after a successful factorisation
 // (gckpp_BatchIntegrator.F90:594-643).
+// Stage S (0-based) after a successful factorisation
+// (gckpp_BatchIntegrator.F90:594-643).
+template <bool M, int S>
+__device__ __forceinline__ void stage(const Blk& b, bool autonomous, double Dir, double H,
+                                      int* ist) {
+  const int Ks = O_K + S * O_VEC;
+  // K_s = Fcn + sum_j HC_j K_j, one pass (the adds in j order, as the
+  // reference's successive WAXPY calls)
+  int src = Ks;
+  if (S == 0 || S == 1) {
+    // stage 1: Fcn = Fcn0; stage 2 has ros_NewF = .FALSE. so Fcn is still Fcn0
+    src = O_FCN0;
+  } else {
+    stage_point<S>(b);
+    fun<M>(b, true, Ks);
+    ist[0] += 1;
+  }
+  double HC[S > 0 ? S : 1];
+#pragma unroll
+  for (int j = 0; j < S; ++j) HC[j] = ros_C[S * (S - 1) / 2 + j] / (Dir * H);
+  struct L {
+    double f, k[S > 0 ? S : 1];
+  };
+  vpass<NVAR>(
+      [&](int k) {
+        L l;
+        l.f = b.ld(src + k * 512);
+#pragma unroll
+        for (int j = 0; j < S; ++j) l.k[j] = b.ld(O_K + j * O_VEC + k * 512);
+        return l;
+      },
+      [&](int k, const L& l) {
+        double v = l.f;
+#pragma unroll
+        for (int j = 0; j < S; ++j) v = v + HC[j] * l.k[j];
+        b.st(Ks + k * 512, v);
+      });
+  if (!autonomous && ros_Gamma[S] != 0.0) {
+    const double HG = (Dir * H) * ros_Gamma[S];
+    struct L2 {
+      double k, d;
+    };
+    vpass<NVAR>([&](int k) { return L2{b.ld(Ks + k * 512), b.ld(O_DFDT + k * 512)}; },
+                [&](int k, const L2& l) { b.st(Ks + k * 512, l.k + HG * l.d); });
+  }
+  solve<M>(b, S);
+  ist[6] += 1;
+}
+
 template <bool M>
 __device__ void stages(const Blk& b, bool autonomous, double Dir, double H, int* ist) {
-  for (int s = 0; s < 4; ++s) {
-    const int Ks = O_K + s * O_VEC;
-    // K_s = Fcn + sum_j HC_j K_j, one pass (the adds in j order, as the
-    // reference's successive WAXPY calls)
-    int src = Ks;
-    if (s == 0 || s == 1) {
-      // stage 1: Fcn = Fcn0; stage 2 has ros_NewF = .FALSE. so Fcn is still Fcn0
-      src = O_FCN0;
-    } else {
-      stage_point(b, s);
-      fun<M>(b, true, Ks);
-      ist[0] += 1;
-    }
-    double HC[3];
-    for (int j = 0; j < s; ++j) HC[j] = ros_C[s * (s - 1) / 2 + j] / (Dir * H);
-    vpass<NVAR>(
-        [&](int k) {
-          double v = b.ld(src + k * 512);
-          for (int j = 0; j < s; ++j) v = v + HC[j] * b.ld(O_K + j * O_VEC + k * 512);
-          return v;
-        },
-        [&](int k, double v) { b.st(Ks + k * 512, v); });
-    if (!autonomous && ros_Gamma[s] != 0.0) {
-      const double HG = (Dir * H) * ros_Gamma[s];
-      vpass<NVAR>([&](int k) { return b.ld(Ks + k * 512) + HG * b.ld(O_DFDT + k * 512); },
-                  [&](int k, double v) { b.st(Ks + k * 512, v); });
-    }
-    solve<M>(b, s);
-    ist[6] += 1;
-  }
+  stage<M, 0>(b, autonomous, Dir, H, ist);
+  stage<M, 1>(b, autonomous, Dir, H, ist);
+  stage<M, 2>(b, autonomous, Dir, H, ist);
+  stage<M, 3>(b, autonomous, Dir, H, ist);
 }
 
 // The stages, then Ynew, Yerr and the error norm
@@ -821,23 +940,28 @@ __device__ double stages_and_error(const Blk& b, const Opts& o, double Dir, doub
   stages<M>(b, o.autonomous, Dir, H, ist);
   double Err = 0.0;
   struct YE {
-    double y, yn, ye;
+    double y, k4[4], at, rt;
   };
   vpass<NVAR>(
       [&](int k) {
-        const double y = b.ld(O_Y + k * 512);
-        double k4[4];
-        for (int j = 0; j < 4; ++j) k4[j] = b.ld(O_K + j * O_VEC + k * 512);
-        double yn = y, ye = 0.0;
-        for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * k4[j];
-        for (int j = 0; j < 4; ++j) ye = ye + ros_E[j] * k4[j];
-        return YE{y, yn, ye};
+        YE v;
+        v.y = b.ld(O_Y + k * 512);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v.k4[j] = b.ld(O_K + j * O_VEC + k * 512);
+        v.at = o.vector_tol ? atol[k] : atol[0];
+        v.rt = o.vector_tol ? rtol[k] : rtol[0];
+        return v;
       },
-      [&](int k, YE v) {  // Err accumulates in element order
-        b.st(O_YNEW + k * 512, v.yn);
-        const double Ymax = dmax(fabs(v.y), fabs(v.yn));
-        const double Scal = o.vector_tol ? atol[k] + rtol[k] * Ymax : atol[0] + rtol[0] * Ymax;
-        const double q = v.ye / Scal;
+      [&](int k, const YE& v) {  // Err accumulates in element order
+        double yn = v.y, ye = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * v.k4[j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ye = ye + ros_E[j] * v.k4[j];
+        b.st(O_YNEW + k * 512, yn);
+        const double Ymax = dmax(fabs(v.y), fabs(yn));
+        const double Scal = v.at + v.rt * Ymax;
+        const double q = ye / Scal;
         Err = Err + q * q;
       });
   Err = sqrt(Err / (double)NVAR);
@@ -846,12 +970,33 @@ __device__ double stages_and_error(const Blk& b, const Opts& o, double Dir, doub
 
 __device__ void accept_y(const Blk& b, const Opts& o) {
   if (o.clamp) {
-    vpass<NVAR>([&](int k) { return dmax(b.ld(O_YNEW + k * 512), 0.0); },
-                [&](int k, double v) { b.st(O_Y + k * 512, v); });
+    vpass<NVAR>([&](int k) { return b.ld(O_YNEW + k * 512); },
+                [&](int k, double v) { b.st(O_Y + k * 512, dmax(v, 0.0)); });
   } else {
     vpass<NVAR>([&](int k) { return b.ld(O_YNEW + k * 512); },
                 [&](int k, double v) { b.st(O_Y + k * 512, v); });
   }
+}
+
+// Y and the four stage vectors of element k, and Ynew = Y + sum_j M_j K_j
+// from them (gckpp_BatchIntegrator.F90:636-640, the stepper's order).
+struct YK {
+  double y, k[4];
+};
+
+__device__ __forceinline__ YK ld_yk(const Blk& b, int k) {
+  YK l;
+  l.y = b.ld(O_Y + k * 512);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) l.k[j] = b.ld(O_K + j * O_VEC + k * 512);
+  return l;
+}
+
+__device__ __forceinline__ double ynew_of(const YK& l) {
+  double yn = l.y;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * l.k[j];
+  return yn;
 }
 
 __device__ __forceinline__ bool time_left(int dir, double T, double Tend) {
@@ -1436,15 +1581,10 @@ __device__ __forceinline__ void adj_update(const Blk& b, double dh) {
 __device__ void adj_step(const Blk& b, double Dir, double H, bool clamp) {
   const double dh = Dir * H;
   if (clamp) {  // Y' = max(Ynew, 0): no sensitivity where Ynew <= 0
-    vpass<NVAR>(
-        [&](int k) {
-          double yn = b.ld(O_Y + k * 512);
-          for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * b.ld(O_K + j * O_VEC + k * 512);
-          return yn;
-        },
-        [&](int k, double yn) {
-          if (!(yn > 0.0)) b.st(O_LAM + k * 512, 0.0);
-        });
+    vpass<NVAR>([&](int k) { return ld_yk(b, k); },
+                [&](int k, const YK& l) {
+                  if (!(ynew_of(l) > 0.0)) b.st(O_LAM + k * 512, 0.0);
+                });
   }
   vpass<NVAR>([&](int k) { return b.ld(O_LAM + k * 512); },
               [&](int k, double l) {
@@ -1473,8 +1613,11 @@ __device__ void adj_step(const Blk& b, double Dir, double H, bool clamp) {
                 [&](int k, double v) { b.st(O_DIR + k * 512, v); });
     adj_gk2(b);
     adj_lp(b, gc_adj_lp2, MECH_NT_LP2);
-    vpass<NVAR>([&](int k) { return b.ld(O_GY + k * 512) + b.ld(O_LP + k * 512); },
-                [&](int k, double v) { b.st(O_GY + k * 512, v); });
+    struct L2 {
+      double g, p;
+    };
+    vpass<NVAR>([&](int k) { return L2{b.ld(O_GY + k * 512), b.ld(O_LP + k * 512)}; },
+                [&](int k, const L2& l) { b.st(O_GY + k * 512, l.g + l.p); });
   }
   vpass<NVAR>([&](int k) { return b.ld(O_GY + k * 512); },
               [&](int k, double v) { b.st(O_LAM + k * 512, v); });
@@ -1578,13 +1721,11 @@ __device__ __forceinline__ void adjoint_cell(const GcAdjArgs& a, unsigned q0, un
         vpass<4 * NVAR>([&](int k) { return b.ld(O_K + k * 512); },
                         [&](int k, double v) { b.st(O_SEGK + (q * 4 * NVAR + k) * 512, v); });
       }
-      vpass<NVAR>(  // Ynew and the accepted update, as the stepper
-          [&](int k) {
-            double yn = b.ld(O_Y + k * 512);
-            for (int j = 0; j < 4; ++j) yn = yn + ros_M[j] * b.ld(O_K + j * O_VEC + k * 512);
-            return clamp ? dmax(yn, 0.0) : yn;
-          },
-          [&](int k, double v) { b.st(O_Y + k * 512, v); });
+      vpass<NVAR>([&](int k) { return ld_yk(b, k); },  // Ynew and the accepted update
+                  [&](int k, const YK& l) {
+                    const double yn = ynew_of(l);
+                    b.st(O_Y + k * 512, clamp ? dmax(yn, 0.0) : yn);
+                  });
     }
     if (ierr != 1) break;
     // the adjoint through the segment's steps, last first: each step's start
