@@ -48,14 +48,20 @@ Tables (int32 words):
   active row segment (at most S entries, plus one spare slot) in LDS; 4-word
   ops: ENTRYG {5, T, 0, 0} + T terms {soff, flags, coef} holding every JVS
   entry of the segment (flags: first/last term of an entry, diagonal, slot
-  index << 8; T a multiple of 4); PIVL {2, slot_j, U(j,j),
-  nupd} and PIVS {3, L(k,j), 0, nupd} + nupd/2 4-word update pairs
-  {t, U(j,c), t, U(j,c)} (nupd a multiple of 4); PIVG {7, G, 0, 0} + G
-  consecutive pivot records of at most 4 updates each, 12 words apiece
-  {2 or 3, slot_j or 0, U(j,j) or L(k,j), 0, t, U, t, U, t, U, t, U}, whose
-  data loads the kernel issues together before applying them in order;
-  STORE {4, n, 0, 0} + n/2 {slot, LU, slot, LU} (n a multiple of 2); END
-  {0,...}.
+  index << 8; T a multiple of 4); the pivot records of the segment (IKJ
+  order: PIVL, a pivot j inside the segment, multiplier -W(j)/U(j,j) stored
+  back as L(k,j); PIVS, a pivot of an earlier segment of the same row,
+  multiplier -L(k,j) read back from LU) cut into sub-records of four
+  updates (a record of more updates continues in sub-records of kind 0 with
+  the same multiplier; padding updates hit the spare slot with U = 0.0),
+  batched as PIVB {8, G, 0, 0} + G packed words {kind | slot_j << 2 |
+  t0 << 8 | t1 << 14 | t2 << 20 | t3 << 26} (kind 1 PIVL, 2 PIVS, 0
+  continuation; LDS slot indices; padded to a multiple of 4 words) + per
+  sub-record {operand, U(j,c0), U(j,c1), U(j,c2), U(j,c3)} (operand U(j,j),
+  L(k,j) or ZERO; padded to a multiple of 4 words), at most PIVB_MAX
+  sub-records per batch: the kernel issues a batch's data loads together,
+  then applies the sub-records in order; STORE {4, n, 0, 0} + n/2 {slot, LU,
+  slot, LU} (n a multiple of 2); END {0,...}.
   (gckpp_Jacobian.F90:3674-20887, gckpp_BatchIntegrator.F90:573-577,
   gckpp_LinearAlgebra.F90:46-84)
 * ``fwd<s>`` (stage vector K_s) and ``tr1``/``tr2`` (the TMP vector): solve
@@ -93,7 +99,7 @@ from pathlib import Path
 SEGMENTS = (16, 24, 32)  # LU row-segment sizes compiled (LDS entries per lane)
 
 
-PIVG_MAX = 3   # pivot records per PIVG group (the kernel has forms for 2 and 3)
+PIVB_MAX = 6   # pivot sub-records per PIVB batch (the kernel has forms for 1..6)
 
 
 def mask_words(n: int) -> int:
@@ -303,24 +309,32 @@ class Tables:
         pad = (spare, self.off("ZERO", 0))
 
         def emit(recs):
-            # runs of pivot records with at most 4 updates become PIVG groups
-            # of up to PIVG_MAX (a record alone stays PIVL/PIVS)
-            out, q = [], 0
-            while q < len(recs):
-                g = 0
-                while q + g < len(recs) and g < PIVG_MAX and len(recs[q + g][3]) <= 4:
-                    g += 1
-                if g >= 2:
-                    out += [7, g, 0, 0]
-                    for kind, f1, opnd, ups in recs[q:q + g]:
-                        full = list(ups) + [pad] * (4 - len(ups))  # exactly 4 updates
-                        out += [kind, f1, opnd, 0] + pairs(full, 4, pad)[1]
-                    q += g
-                    continue
-                kind, f1, opnd, ups = recs[q]
-                n, w = pairs(ups, 4, pad)
-                out += ([2, f1, opnd, n] if kind == 2 else [3, opnd, 0, n]) + w
-                q += 1
+            subs = []  # (kind, slot_j, operand, four (t, U) updates)
+            for kind, f1, opnd, ups in recs:
+                ups = list(ups)
+                first = True
+                while first or ups:
+                    chunk, ups = ups[:4], ups[4:]
+                    chunk += [pad] * (4 - len(chunk))
+                    if first:
+                        subs.append((1, f1, opnd, chunk) if kind == 2 else (2, 0, opnd, chunk))
+                    else:
+                        subs.append((0, 0, self.off("ZERO", 0), chunk))
+                    first = False
+            out = []
+            for q in range(0, len(subs), PIVB_MAX):
+                grp = subs[q:q + PIVB_MAX]
+                packed, data = [], []
+                for kind, sj, opnd, chunk in grp:
+                    w = kind | (sj // 512) << 2
+                    for k, (t, u) in enumerate(chunk):
+                        assert t // 512 < 64
+                        w |= (t // 512) << (8 + 6 * k)
+                    packed.append(w - (1 << 32) if w >= 1 << 31 else w)
+                    data += [opnd] + [u for _, u in chunk]
+                packed += [0] * (-len(packed) % 4)
+                data += [0] * (-len(data) % 4)
+                out += [8, len(grp), 0, 0] + packed + data
             return out
 
         for k in range(nvar):

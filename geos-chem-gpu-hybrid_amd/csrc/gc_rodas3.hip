@@ -367,47 +367,53 @@ __device__ __forceinline__ void entry_term(const Blk& b, i4 t, double x, double 
   }
 }
 
-// G consecutive pivot records of at most four updates each (a PIVG group):
-// every record's data loads (its pivot operand and its four U values) are
-// issued together, then the records are applied in order, each one's
-// multiplier read from the LDS row segment after the previous one's updates.
-// Sub-record words {kind, slot_j, operand, 0, t, U, t, U, t, U, t, U}: kind 2
-// a = -W(j)/U(j,j), W(j) = -a (PIVL); kind 3 a = -L(k,j) (PIVS).
+// A PIVB batch of G pivot sub-records of four updates each: header {8, G, 0,
+// 0}, G packed words {kind | sj << 2 | t0 << 8 | t1 << 14 | t2 << 20 |
+// t3 << 26} (LDS slot indices; padded to a multiple of 4 words), then per
+// sub-record {operand, U0, U1, U2, U3} (padded to a multiple of 4 words).
+// Every data load of the batch is issued before the first sub-record is
+// applied; sub-records then go in order, each reading its multiplier's inputs
+// from the LDS row segment after the previous one's updates: kind 1 (PIVL)
+// a = -W(j)/U(j,j) with W(j) = -a stored back, kind 2 (PIVS) a = -L(k,j),
+// kind 0 continues the previous sub-record's multiplier (a record of more
+// than four updates).  W(t_k) += a*U_k; padding updates hit the spare slot
+// with U = 0.0.
 template <int G>
-__device__ __forceinline__ ctab_t* piv_group(const Blk& b, ctab_t* op) {
-  i4 h[G], p01[G], p23[G];
+__device__ __forceinline__ ctab_t* piv_batch(const Blk& b, ctab_t* op, double& a) {
+  constexpr int NPK = (G + 3) / 4, ND = (5 * G + 3) / 4;
+  i4 pk[NPK], dw[ND];
+#pragma unroll
+  for (int i = 0; i < NPK; ++i) pk[i] = ld4(op + 4 + 4 * i);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) dw[i] = ld4(op + 4 + 4 * NPK + 4 * i);
+  auto word = [&](const i4* q, int k) {
+    const i4 v = q[k >> 2];
+    return (k & 3) == 0 ? v.x : (k & 3) == 1 ? v.y : (k & 3) == 2 ? v.z : v.w;
+  };
+  double v[5 * G];
+#pragma unroll
+  for (int k = 0; k < 5 * G; ++k) v[k] = b.ld(word(dw, k));
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    h[g] = ld4(op + 12 * g);
-    p01[g] = ld4(op + 12 * g + 4);
-    p23[g] = ld4(op + 12 * g + 8);
-  }
-  double v[G], u[G][4];
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    v[g] = b.ld(h[g].z);
-    u[g][0] = b.ld(p01[g].y);
-    u[g][1] = b.ld(p01[g].w);
-    u[g][2] = b.ld(p23[g].y);
-    u[g][3] = b.ld(p23[g].w);
-  }
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    double a;
-    if (h[g].x == 2) {
-      a = -b.lds_ld(h[g].y) / v[g];
-      b.lds_st(h[g].y, -a);
-    } else {
-      a = -v[g];
+    const unsigned p = (unsigned)word(pk, g);
+    const unsigned kind = p & 3u;
+    if (kind == 1u) {
+      const int sj = (int)((p >> 2) & 63u) * 512;
+      a = -b.lds_ld(sj) / v[5 * g];
+      b.lds_st(sj, -a);
+    } else if (kind == 2u) {
+      a = -v[5 * g];
     }
-    const double w0 = b.lds_ld(p01[g].x), w1 = b.lds_ld(p01[g].z);
-    const double w2 = b.lds_ld(p23[g].x), w3 = b.lds_ld(p23[g].z);
-    b.lds_st(p01[g].x, w0 + a * u[g][0]);
-    b.lds_st(p01[g].z, w1 + a * u[g][1]);
-    b.lds_st(p23[g].x, w2 + a * u[g][2]);
-    b.lds_st(p23[g].z, w3 + a * u[g][3]);
+    const int t0 = (int)((p >> 8) & 63u) * 512, t1 = (int)((p >> 14) & 63u) * 512;
+    const int t2 = (int)((p >> 20) & 63u) * 512, t3 = (int)(p >> 26) * 512;
+    const double w0 = b.lds_ld(t0), w1 = b.lds_ld(t1), w2 = b.lds_ld(t2), w3 = b.lds_ld(t3);
+    b.lds_st(t0, w0 + a * v[5 * g + 1]);
+    b.lds_st(t1, w1 + a * v[5 * g + 2]);
+    b.lds_st(t2, w2 + a * v[5 * g + 3]);
+    b.lds_st(t3, w3 + a * v[5 * g + 4]);
   }
-  return op + 12 * G;
+  return op + 4 + 4 * NPK + 4 * ND;
 }
 
 // Jac_SP fused with Ghimj = -Jac0, diag += ghinv and KppDecomp, row segments
@@ -418,18 +424,30 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
   run_products(b, gc_prod_bjac, gc_prodgrp_bjac);
   ctab_t* op = tab(GC_LU_TABLE);
   int sing = 0;
+  double a = 0.0;  // the current pivot multiplier (carried across PIVB batches)
   MaskWord mj;
   for (;;) {
     const i4 h = ld4(op), f = ld4(op + 4);  // header and the first 4 words after it
     const int code = h.x;
     if (code == 0) break;
     if (code == 5) {
-      // every entry of the segment: T terms (a multiple of 4), eight or four
-      // loads in flight at a time; an entry may span batches
+      // every entry of the segment: T terms (a multiple of 4), sixteen, eight
+      // or four loads in flight at a time; an entry may span batches
       const int T = h.y;
       op += 4;
       double acc = 0.0;
       int q = 0;
+      for (; q + 16 <= T; q += 16, op += 64) {
+        i4 tt[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) tt[r] = ld4(op + 4 * r);
+        double x[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) x[r] = b.ld(tt[r].x);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) entry_term<M>(b, tt[r], x[r], ghinv, acc, sing, mj);
+      }
       for (; q + 8 <= T; q += 8, op += 32) {
         i4 tt[8];
 #pragma unroll
@@ -437,6 +455,7 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
         double x[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) x[r] = b.ld(tt[r].x);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int r = 0; r < 8; ++r) entry_term<M>(b, tt[r], x[r], ghinv, acc, sing, mj);
       }
@@ -450,96 +469,14 @@ __device__ int jac_lu(const Blk& b, double ghinv) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) entry_term<M>(b, tt[r], x[r], ghinv, acc, sing, mj);
       }
-    } else if (code == 7) {
-      op = (h.y == 3) ? piv_group<3>(b, op + 4) : piv_group<2>(b, op + 4);
-    } else if (code == 2 || code == 3) {
-      // n updates {slot, U} (n a multiple of 4; padding entries update the
-      // spare slot, real ones have distinct slots): the first four are loaded
-      // together with the pivot operand, then sixteen, eight, four per iteration
-      const int n = h.w;
-      op += 4;
-      i4 p01 = f, p23 = ld4(op + 4);
-      double u0 = 0.0, u1 = 0.0, u2 = 0.0, u3 = 0.0;
-      if (n > 0) {
-        u0 = b.ld(p01.y);
-        u1 = b.ld(p01.w);
-        u2 = b.ld(p23.y);
-        u3 = b.ld(p23.w);
-      }
-      double a;
-      if (code == 2) {
-        const int sj = h.y;
-        a = -b.lds_ld(sj) / b.ld(h.z);
-        b.lds_st(sj, -a);
-      } else {
-        a = -b.ld(h.y);
-      }
-      int u = 0;
-      if (n > 0) {
-        const double w0 = b.lds_ld(p01.x), w1 = b.lds_ld(p01.z);
-        const double w2 = b.lds_ld(p23.x), w3 = b.lds_ld(p23.z);
-        b.lds_st(p01.x, w0 + a * u0);
-        b.lds_st(p01.z, w1 + a * u1);
-        b.lds_st(p23.x, w2 + a * u2);
-        b.lds_st(p23.z, w3 + a * u3);
-        u = 4;
-        op += 8;
-      }
-      for (; u + 16 <= n; u += 16, op += 32) {
-        i4 pp[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) pp[r] = ld4(op + 4 * r);
-        double uu[16], ww[16];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          uu[2 * r] = b.ld(pp[r].y);
-          uu[2 * r + 1] = b.ld(pp[r].w);
-        }
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          ww[2 * r] = b.lds_ld(pp[r].x);
-          ww[2 * r + 1] = b.lds_ld(pp[r].z);
-        }
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          b.lds_st(pp[r].x, ww[2 * r] + a * uu[2 * r]);
-          b.lds_st(pp[r].z, ww[2 * r + 1] + a * uu[2 * r + 1]);
-        }
-      }
-      for (; u + 8 <= n; u += 8, op += 16) {
-        i4 pp[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) pp[r] = ld4(op + 4 * r);
-        double uu[8], ww[8];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          uu[2 * r] = b.ld(pp[r].y);
-          uu[2 * r + 1] = b.ld(pp[r].w);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          ww[2 * r] = b.lds_ld(pp[r].x);
-          ww[2 * r + 1] = b.lds_ld(pp[r].z);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          b.lds_st(pp[r].x, ww[2 * r] + a * uu[2 * r]);
-          b.lds_st(pp[r].z, ww[2 * r + 1] + a * uu[2 * r + 1]);
-        }
-      }
-      for (; u < n; u += 4, op += 8) {
-        p01 = ld4(op);
-        p23 = ld4(op + 4);
-        u0 = b.ld(p01.y);
-        u1 = b.ld(p01.w);
-        u2 = b.ld(p23.y);
-        u3 = b.ld(p23.w);
-        const double w0 = b.lds_ld(p01.x), w1 = b.lds_ld(p01.z);
-        const double w2 = b.lds_ld(p23.x), w3 = b.lds_ld(p23.z);
-        b.lds_st(p01.x, w0 + a * u0);
-        b.lds_st(p01.z, w1 + a * u1);
-        b.lds_st(p23.x, w2 + a * u2);
-        b.lds_st(p23.z, w3 + a * u3);
+    } else if (code == 8) {
+      switch (h.y) {
+        case 1: op = piv_batch<1>(b, op, a); break;
+        case 2: op = piv_batch<2>(b, op, a); break;
+        case 3: op = piv_batch<3>(b, op, a); break;
+        case 4: op = piv_batch<4>(b, op, a); break;
+        case 5: op = piv_batch<5>(b, op, a); break;
+        default: op = piv_batch<6>(b, op, a); break;
       }
     } else {  // code 4: store the segment, pairs {slot, LU} (padding writes the sink)
       const int n = h.y;

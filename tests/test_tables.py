@@ -169,37 +169,26 @@ class TableMachine:
                             sing |= np.abs(g) < 2.2250738585072014e-308
                         self.lds[(fl >> 8) & 63] = g
                     p += 4
-            elif code == 7:
+            elif code == 8:
                 G = w[p + 1]
-                assert 2 <= G <= 3
-                p += 4
-                for _ in range(G):
-                    kind, sj, opnd = w[p], w[p + 1], w[p + 2]
-                    if kind == 2:
-                        a = -lds(sj) / self.ld(opnd)
+                assert 1 <= G <= GT.PIVB_MAX
+                npk = -(-G // 4) * 4
+                pk = w[p + 4:p + 4 + G]
+                d = w[p + 4 + npk:p + 4 + npk + 5 * G]
+                p += 4 + npk + -(-5 * G // 4) * 4
+                for g in range(G):
+                    word = pk[g] & 0xFFFFFFFF
+                    kind, sj = word & 3, ((word >> 2) & 63) * 512
+                    if kind == 1:
+                        a = -lds(sj) / self.ld(d[5 * g])
                         self.lds[sj // 512] = -a
+                    elif kind == 2:
+                        a = -self.ld(d[5 * g])
                     else:
-                        assert kind == 3
-                        a = -self.ld(opnd)
-                    p += 4
-                    for _ in range(4):
-                        t, u = w[p], w[p + 1]
-                        self.lds[t // 512] = lds(t) + a * self.ld(u)
-                        p += 2
-            elif code in (2, 3):
-                if code == 2:
-                    sj = w[p + 1]
-                    a = -lds(sj) / self.ld(w[p + 2])
-                    self.lds[sj // 512] = -a
-                else:
-                    a = -self.ld(w[p + 1])
-                n = w[p + 3]
-                assert n % 4 == 0
-                p += 4
-                for _ in range(n):
-                    t, u = w[p], w[p + 1]
-                    self.lds[t // 512] = lds(t) + a * self.ld(u)
-                    p += 2
+                        assert kind == 0  # continues the previous sub-record's multiplier
+                    for k in range(4):
+                        t = ((word >> (8 + 6 * k)) & 63) * 512
+                        self.lds[t // 512] = lds(t) + a * self.ld(d[5 * g + 1 + k])
             else:
                 assert code == 4
                 n = w[p + 1]
