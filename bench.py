@@ -201,16 +201,19 @@ def resident_rate(d):
         r.close()
 
 
-def membench(block_doubles, waves_per_cu, loads_per_store):
+def membench(block_doubles, waves_per_cu, loads_per_store, loads_in_flight=16):
     """Memory ceiling of the workspace access shape on this device
     (csrc/gc_membench.hip, built by build.py): random 512-byte wave accesses
-    in per-wave blocks of the solver's size and read/write mix."""
+    in per-wave blocks of the solver's size and read/write mix, 16 reads in
+    flight per wave (where the shape's rate saturates), and the plain
+    streaming read rate of the same buffer."""
     import subprocess
     exe = PKG / "lib" / "gc_membench"
     if not exe.exists():
         log("gc_membench is not built: no measured ceiling")
         return None
-    out = subprocess.run([str(exe), str(block_doubles), str(waves_per_cu), str(loads_per_store)],
+    out = subprocess.run([str(exe), str(block_doubles), str(waves_per_cu), str(loads_per_store),
+                          str(loads_in_flight)],
                          check=True, capture_output=True, text=True, timeout=120).stdout
     return json.loads(out.strip().splitlines()[-1])
 
@@ -483,9 +486,13 @@ def main():
                                 "frac_of_pattern_ceiling": round(gbps / ceil, 4),
                                 "pattern": f"random 512-byte wave accesses in per-wave blocks "
                                            f"of {mb['block_doubles']} doubles/lane, "
-                                           f"{mb['waves_per_cu']} waves/CU, 1 write per "
-                                           f"{mb['loads_per_store']} reads (gc_membench)",
-                                "stream_read": round(mb["stream_read_tbs"] * 1e3, 1)})
+                                           f"{mb['waves_per_cu']} waves/CU, "
+                                           f"{mb['loads_in_flight']} reads in flight per wave, "
+                                           f"1 write per {mb['loads_per_store']} reads "
+                                           f"(gc_membench)",
+                                "stream_read": round(mb["stream_read_tbs"] * 1e3, 1),
+                                "frac_of_stream_read": round(gbps / (mb["stream_read_tbs"] * 1e3),
+                                                             4)})
             log(f"membench: {mb}")
 
     res = {

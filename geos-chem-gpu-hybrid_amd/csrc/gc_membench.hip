@@ -7,15 +7,17 @@
 // program measures what the memory system delivers for exactly that shape on
 // the device it runs on:
 //   stream   plain streaming read of the whole buffer (all waves, grid-stride)
-//   gather   random 512-byte wave reads, 8 in flight per wave, with one
-//            512-byte wave write per `loads_per_store` reads
+//   gather   random 512-byte wave reads, `loads_in_flight` (4, 8, 16 or 32)
+//            in flight per wave, with one 512-byte wave write per
+//            `loads_per_store` reads
 //   chain    latency of one 512-byte wave read whose address depends on the
 //            previous read (one wave per CU, each in its own block): the
 //            round trip a chain of dependent operations pays per link
 // for a given block size (doubles per lane) and waves per CU; the working set
 // of the gather is CUs * waves_per_cu * block_doubles * 512 bytes, so small
 // blocks measure the shape out of the on-chip caches.  It prints one JSON
-// line.  Usage: gc_membench [block_doubles waves_per_cu loads_per_store]
+// line.  Usage:
+//   gc_membench [block_doubles waves_per_cu loads_per_store loads_in_flight]
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
@@ -23,8 +25,7 @@
 
 namespace {
 
-constexpr int kLoads = 8;
-
+template <int kLoads>
 __global__ void __launch_bounds__(256) k_gather(double* buf, long block_elems, int iters,
                                                int loads_per_store, double* out) {
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -95,7 +96,9 @@ int main(int argc, char** argv) {
   long block = argc > 1 ? atol(argv[1]) : 12799;
   int wpc = argc > 2 ? atoi(argv[2]) : 8;
   int lps = argc > 3 ? atoi(argv[3]) : 5;
-  if (block < 64 || block > 65536 || wpc < 4 || wpc > 32 || wpc % 4 || lps < 0) {
+  int kl = argc > 4 ? atoi(argv[4]) : 16;
+  if (block < 64 || block > 65536 || wpc < 4 || wpc > 32 || wpc % 4 || lps < 0 ||
+      (kl != 4 && kl != 8 && kl != 16 && kl != 32)) {
     fprintf(stderr, "gc_membench: bad arguments\n");
     return 2;
   }
@@ -123,16 +126,24 @@ int main(int argc, char** argv) {
   hipEventElapsedTime(&ms, a, b);
   const double stream_tbs = n * 8.0 / (ms * 1e-3) / 1e12;
 
-  const int iters = 1000;
+  const int iters = 8000 / kl;  // 8,000 reads per wave whatever the depth
   const int grid = (int)(waves * 64 / 256);
-  hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, 0, buf, block, 4, lps, out);
+  auto gather = [&](int it) {
+    switch (kl) {
+      case 4: hipLaunchKernelGGL(k_gather<4>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, out); break;
+      case 8: hipLaunchKernelGGL(k_gather<8>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, out); break;
+      case 16: hipLaunchKernelGGL(k_gather<16>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, out); break;
+      default: hipLaunchKernelGGL(k_gather<32>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, out); break;
+    }
+  };
+  gather(1);
   hipEventRecord(a, 0);
-  hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, 0, buf, block, iters, lps, out);
+  gather(iters);
   hipEventRecord(b, 0);
   if (!ok(hipEventSynchronize(b), "gather")) return 1;
   hipEventElapsedTime(&ms, a, b);
-  const double loads = (double)waves * iters * kLoads;
-  const double stores = lps > 0 ? (double)waves * ((double)iters * kLoads / lps) : 0.0;
+  const double loads = (double)waves * iters * kl;
+  const double stores = lps > 0 ? (double)waves * ((double)iters * kl / lps) : 0.0;
   const double gather_tbs = (loads + stores) * 512.0 / (ms * 1e-3) / 1e12;
 
   // the chain: one wave per CU, blocks 0..cus-1 of the buffer (cus <= waves)
@@ -145,9 +156,10 @@ int main(int argc, char** argv) {
   hipEventElapsedTime(&ms, a, b);
   const double chain_ns = ms * 1e6 / links;
   printf("{\"device\": \"%s\", \"cus\": %d, \"block_doubles\": %ld, \"waves_per_cu\": %d, "
-         "\"loads_per_store\": %d, \"buffer_gb\": %.2f, \"stream_read_tbs\": %.3f, "
+         "\"loads_per_store\": %d, \"loads_in_flight\": %d, \"buffer_gb\": %.2f, "
+         "\"stream_read_tbs\": %.3f, "
          "\"gather_tbs\": %.3f, \"chain_ns\": %.1f, \"chain_set_mb\": %.1f}\n",
-         prop.gcnArchName, cus, block, wpc, lps, n * 8.0 / 1e9, stream_tbs, gather_tbs, chain_ns,
+         prop.gcnArchName, cus, block, wpc, lps, kl, n * 8.0 / 1e9, stream_tbs, gather_tbs, chain_ns,
          cus * block * 512.0 / 1e6);
   hipFree(buf);
   hipFree(out);
