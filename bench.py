@@ -98,7 +98,7 @@ def cpu_baseline(mech, d, perm, target_s=15.0):
     n = int(min(n_all, max(n_cal, n_cal * target_s / max(t_cal, 1e-3))))
     log(f"cpu baseline: calibration {n_cal} cells in {t_cal:.2f} s -> sample of {n} cells")
     secs = run(n)
-    if secs < 0.6 * target_s and n < n_all:  # the small calibration overestimates the rate
+    if secs < 10.0 and n < n_all:  # the small calibration overestimates the rate
         n = int(min(n_all, n * target_s / max(secs, 1e-3)))
         log(f"cpu baseline: {secs:.1f} s was short -> sample of {n} cells")
         secs = run(n)
