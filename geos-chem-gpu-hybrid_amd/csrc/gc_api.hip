@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -86,6 +87,31 @@ Tuning current_tuning() {
   return g_tune;
 }
 
+// ---- overlapped upload of the host-array entry point (process-wide) -----------
+// mode 1: a call whose launch order is the identity (the model gathers its
+// batch in cost order and passes Perm = 1..nCell) uploads its inputs in chunks
+// of launch positions while the persistent kernel already runs, each wave
+// waiting for its positions' chunk (GcGate); mode 0: upload, then solve;
+// mode 2: as mode 1, but the flags are never set, so every wait runs to its
+// bound (the failure path, for tests).  cells = chunk size in cells (0: what
+// fits one 32 MB staging slot); spin = sleeps of ~3.4 us a wave waits for a
+// chunk.  A mode-1 call whose chunks do not land within the bound (copies
+// that cannot run beside the kernel) switches the overlap off for the rest
+// of the process and is solved again with the upload first; in mode 2 the
+// call fails (IERR -98).
+struct GateConfig {
+  int mode = 1;
+  int cells = 0;
+  unsigned long long spin = 600000ull;  // ~2 s; the whole 4x5 upload takes ~70 ms
+};
+std::mutex g_gate_mu;
+GateConfig g_gate;
+
+GateConfig current_gate() {
+  std::lock_guard<std::mutex> lk(g_gate_mu);
+  return g_gate;
+}
+
 // ---- per-device context -----------------------------------------------------
 struct DevBuf {
   void* p = nullptr;
@@ -111,10 +137,12 @@ struct DevBuf {
 };
 
 // ---- pinned staging of host arrays --------------------------------------------
-// Host arrays move through pinned buffers in chunks: two host threads copy
+// Host arrays move through pinned buffers in chunks: four host threads copy
 // chunks into (or out of) their own pair of pinned buffers while the DMA
-// engine moves the chunk before, so the host copy and the transfer overlap
-// and the transfer runs at the pinned rate.  A device without pinned buffers
+// engines move the chunks before, so the host copies and the transfers
+// overlap and the transfers run at the pinned rate.  The threads share two
+// streams (thread t issues on stream t % 2), so staging adds two streams to
+// the process whatever the thread count.  A device without pinned buffers
 // (hipHostMalloc failed) falls back to plain pageable copies.
 struct Piece {
   void* dev;
@@ -123,20 +151,23 @@ struct Piece {
 };
 
 struct Stager {
-  static constexpr int kThreads = 2, kSlots = 2;
+  static constexpr int kThreads = 4, kStreams = 2, kSlots = 2;
   static constexpr size_t kChunk = size_t(32) << 20;
   bool tried = false, ok = false;
   unsigned char* pin[kThreads][kSlots] = {};
-  hipStream_t st[kThreads] = {};
+  unsigned* ones = nullptr;  // pinned 1s: the source of the input-gate flags
+  hipStream_t st[kStreams] = {};
   hipEvent_t ev[kThreads][kSlots] = {};
-  hipEvent_t done[kThreads] = {};
+  hipEvent_t done[kStreams] = {};
 
   void init() {
     if (tried) return;
     tried = true;
+    for (int i = 0; i < kStreams; ++i) {
+      if (hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking) != hipSuccess) return;
+      if (hipEventCreateWithFlags(&done[i], hipEventDisableTiming) != hipSuccess) return;
+    }
     for (int t = 0; t < kThreads; ++t) {
-      if (hipStreamCreateWithFlags(&st[t], hipStreamNonBlocking) != hipSuccess) return;
-      if (hipEventCreateWithFlags(&done[t], hipEventDisableTiming) != hipSuccess) return;
       for (int k = 0; k < kSlots; ++k) {
         if (hipHostMalloc((void**)&pin[t][k], kChunk, hipHostMallocDefault) != hipSuccess) {
           pin[t][k] = nullptr;
@@ -146,7 +177,34 @@ struct Stager {
         if (hipEventCreateWithFlags(&ev[t][k], hipEventDisableTiming) != hipSuccess) return;
       }
     }
+    if (hipHostMalloc((void**)&ones, sizeof(unsigned), hipHostMallocDefault) != hipSuccess) {
+      ones = nullptr;
+      (void)hipGetLastError();
+      return;
+    }
+    *ones = 1u;
     ok = true;
+  }
+
+  hipStream_t stream_of(int t) const { return st[t % kStreams]; }
+
+  // work(t) on every thread t (thread 0 is the caller's); the first error
+  static hipError_t run(const std::function<hipError_t(int)>& work) {
+    hipError_t err[kThreads];
+    std::vector<std::thread> helpers;
+    for (int t = 1; t < kThreads; ++t) helpers.emplace_back([&, t] { err[t] = work(t); });
+    err[0] = work(0);
+    for (auto& h : helpers) h.join();
+    for (int t = 0; t < kThreads; ++t)
+      if (err[t] != hipSuccess) return err[t];
+    return hipSuccess;
+  }
+
+  // the staging streams start once `ev_start` has been reached
+  hipError_t wait_for(hipEvent_t ev_start) {
+    for (int i = 0; i < kStreams; ++i)
+      if (hipStreamWaitEvent(st[i], ev_start, 0) != hipSuccess) return hipErrorUnknown;
+    return hipSuccess;
   }
 
   // chunk c of the pieces -> (piece, offset, length)
@@ -170,32 +228,71 @@ struct Stager {
       return hipSuccess;
     }
     const std::vector<Piece> c = chunks(p, n);
-    hipError_t err[kThreads] = {hipSuccess, hipSuccess};
-    if (hipEventRecord(done[0], after) != hipSuccess) return hipErrorUnknown;
-    for (int t = 0; t < kThreads; ++t)
-      if (hipStreamWaitEvent(st[t], done[0], 0) != hipSuccess) return hipErrorUnknown;
-    auto work = [&](int t) {
+    if (hipEventRecord(done[0], after) != hipSuccess || wait_for(done[0]) != hipSuccess)
+      return hipErrorUnknown;
+    hipError_t r = run([&](int t) {
       hipError_t e = hipSetDevice(dev);
       for (size_t k = t, j = 0; k < c.size() && e == hipSuccess; k += kThreads, ++j) {
         const int s = (int)(j % kSlots);
         e = hipEventSynchronize(ev[t][s]);  // the slot's previous transfer is done
         if (e != hipSuccess) break;
         memcpy(pin[t][s], c[k].host, c[k].bytes);
-        e = hipMemcpyAsync(c[k].dev, pin[t][s], c[k].bytes, hipMemcpyHostToDevice, st[t]);
-        if (e == hipSuccess) e = hipEventRecord(ev[t][s], st[t]);
+        e = hipMemcpyAsync(c[k].dev, pin[t][s], c[k].bytes, hipMemcpyHostToDevice, stream_of(t));
+        if (e == hipSuccess) e = hipEventRecord(ev[t][s], stream_of(t));
       }
-      err[t] = e;
-    };
-    std::thread helper(work, 1);
-    work(0);
-    helper.join();
-    for (int t = 0; t < kThreads; ++t) {
-      if (err[t] != hipSuccess) return err[t];
-      hipError_t e = hipEventRecord(done[t], st[t]);
-      if (e == hipSuccess) e = hipStreamWaitEvent(consumer, done[t], 0);
+      return e;
+    });
+    if (r != hipSuccess) return r;
+    for (int i = 0; i < kStreams; ++i) {
+      hipError_t e = hipEventRecord(done[i], st[i]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(consumer, done[i], 0);
       if (e != hipSuccess) return e;
     }
     return hipSuccess;
+  }
+
+  // Host -> device of a batch in chunks of `cells` consecutive cells (launch
+  // positions) behind an input gate: chunk k of every piece (piece i holds
+  // stride[i] bytes per cell) goes through one pinned slot, then flags[k] is
+  // set to 1 on the same stream, i.e. once the chunk's data has landed
+  // (unless `signal` is false: the flags are then never set).  The streams
+  // start once `start` has been reached; the call returns when every copy is
+  // issued, and sync() waits for them.
+  hipError_t to_device_gated(int dev, const Piece* p, const size_t* stride, int n, int nCell,
+                             int cells, unsigned* flags, bool signal, hipEvent_t start) {
+    const int K = (nCell + cells - 1) / cells;
+    if (wait_for(start) != hipSuccess) return hipErrorUnknown;
+    return run([&](int t) {
+      hipError_t e = hipSetDevice(dev);
+      for (int k = t, j = 0; k < K && e == hipSuccess; k += kThreads, ++j) {
+        const int s = j % kSlots;
+        e = hipEventSynchronize(ev[t][s]);  // the slot's previous transfer is done
+        const size_t c0 = (size_t)k * (size_t)cells;
+        const size_t m = std::min((size_t)cells, (size_t)nCell - c0);
+        size_t off = 0;
+        for (int i = 0; i < n && e == hipSuccess; ++i) {
+          const size_t bytes = m * stride[i];
+          memcpy(pin[t][s] + off, (const char*)p[i].host + c0 * stride[i], bytes);
+          e = hipMemcpyAsync((char*)p[i].dev + c0 * stride[i], pin[t][s] + off, bytes,
+                             hipMemcpyHostToDevice, stream_of(t));
+          off += bytes;
+        }
+        if (e == hipSuccess && signal)
+          e = hipMemcpyAsync(flags + k, ones, sizeof(unsigned), hipMemcpyHostToDevice,
+                             stream_of(t));
+        if (e == hipSuccess) e = hipEventRecord(ev[t][s], stream_of(t));
+      }
+      return e;
+    });
+  }
+
+  hipError_t sync() {
+    hipError_t r = hipSuccess;
+    for (int i = 0; i < kStreams; ++i) {
+      hipError_t e = hipStreamSynchronize(st[i]);
+      if (r == hipSuccess) r = e;
+    }
+    return r;
   }
 
   // Device -> host after the work queued on `producer`; returns when the
@@ -210,19 +307,17 @@ struct Stager {
       return hipStreamSynchronize(producer);
     }
     const std::vector<Piece> c = chunks(p, n);
-    hipError_t err[kThreads] = {hipSuccess, hipSuccess};
-    if (hipEventRecord(done[0], producer) != hipSuccess) return hipErrorUnknown;
-    for (int t = 0; t < kThreads; ++t)
-      if (hipStreamWaitEvent(st[t], done[0], 0) != hipSuccess) return hipErrorUnknown;
-    auto work = [&](int t) {
+    if (hipEventRecord(done[0], producer) != hipSuccess || wait_for(done[0]) != hipSuccess)
+      return hipErrorUnknown;
+    return run([&](int t) {
       hipError_t e = hipSetDevice(dev);
       std::vector<size_t> mine;
       for (size_t k = t; k < c.size(); k += kThreads) mine.push_back(k);
       auto issue = [&](size_t j) {
         const int s = (int)(j % kSlots);
         hipError_t r = hipMemcpyAsync(pin[t][s], c[mine[j]].dev, c[mine[j]].bytes,
-                                      hipMemcpyDeviceToHost, st[t]);
-        if (r == hipSuccess) r = hipEventRecord(ev[t][s], st[t]);
+                                      hipMemcpyDeviceToHost, stream_of(t));
+        if (r == hipSuccess) r = hipEventRecord(ev[t][s], stream_of(t));
         return r;
       };
       if (e == hipSuccess && !mine.empty()) e = issue(0);
@@ -231,14 +326,8 @@ struct Stager {
         if (e == hipSuccess) e = hipEventSynchronize(ev[t][j % kSlots]);
         if (e == hipSuccess) memcpy(c[mine[j]].host, pin[t][j % kSlots], c[mine[j]].bytes);
       }
-      err[t] = e;
-    };
-    std::thread helper(work, 1);
-    work(0);
-    helper.join();
-    for (int t = 0; t < kThreads; ++t)
-      if (err[t] != hipSuccess) return err[t];
-    return hipSuccess;
+      return e;
+    });
   }
 };
 
@@ -291,6 +380,7 @@ struct DevCtx {
   Stager stage;                      // pinned staging of the host-array entry points
   // staging for the host-array entry points
   DevBuf C, RCONST, ATOL, RTOL, ICNTRL, RCNTRL, Perm, ISTAT, RSTAT, IERR;
+  DevBuf gate;                 // input-gate flags of the overlapped upload, then its error word
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_in = nullptr;  // orders the library stream after the caller's
 };
@@ -616,6 +706,125 @@ DevCtx* enter(void* stream, int* rc) {
 __global__ void k_iota1(int* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i + 1;
+}
+
+// The host-array solve with the inputs uploaded behind the input gate (see
+// GateConfig): the device's buffers are sized, the flags cleared, ATOL, RTOL
+// and the identity launch order set, the gated persistent kernel launched,
+// then the chunks of C, RCONST, ICNTRL and RCNTRL (launch positions = cells)
+// copied while it runs.  Returns 0, or -2 / -3 (allocation / runtime error)
+// or -4 (a chunk's flag was not seen within the bound).
+int gated_solve(DevCtx& c, int dev, const GateConfig& g, int nCell, double Tstart, double Tend,
+                const double* C, const double* RCONST, const double* ATOL, const double* RTOL,
+                const int* ICNTRL, const double* RCNTRL, double* tKernel) {
+  *tKernel = 0.0;
+  const MechDims md = mech_dims();
+  const size_t n = (size_t)nCell;
+  const size_t nspec = (size_t)(md.nvar + md.nfix);
+  const size_t stride[4] = {nspec * 8, (size_t)md.nreact * 8, 20 * 4, 20 * 8};
+  const size_t per_cell = stride[0] + stride[1] + stride[2] + stride[3];
+  // chunks of a multiple of 64 cells: every chunk boundary of every array then
+  // falls on a 128-byte line boundary (the arrays' per-cell sizes are multiples
+  // of 8 bytes), so no line holds both a landed cell and one still in flight
+  int cells = g.cells > 0 ? g.cells : (int)(Stager::kChunk / per_cell);
+  if ((size_t)cells * per_cell > Stager::kChunk) cells = (int)(Stager::kChunk / per_cell);
+  cells -= cells % 64;
+  if (cells < 64) cells = 64;
+  const int K = (nCell + cells - 1) / cells;
+  const Tuning t = current_tuning();
+  const unsigned S = slots_for(c, GC_AMD_KERNEL_PERSISTENT, t, nCell);
+  hipError_t e = hipSuccess;
+  DevBuf* bufs[] = {&c.C, &c.RCONST, &c.ICNTRL, &c.RCNTRL};
+  for (int i = 0; i < 4 && e == hipSuccess; ++i) e = bufs[i]->ensure(n * stride[i]);
+  if (e == hipSuccess) e = c.ATOL.ensure((size_t)md.nvar * 8);
+  if (e == hipSuccess) e = c.RTOL.ensure((size_t)md.nvar * 8);
+  if (e == hipSuccess) e = c.Perm.ensure(n * 4);
+  if (e == hipSuccess) e = c.ISTAT.ensure(n * 8 * 4);
+  if (e == hipSuccess) e = c.RSTAT.ensure(n * 3 * 8);
+  if (e == hipSuccess) e = c.IERR.ensure(n * 4);
+  if (e == hipSuccess) e = c.gate.ensure(((size_t)K + 1) * sizeof(unsigned));
+  if (e == hipSuccess) e = c.ws.ensure((size_t)S * (size_t)ws_per_slot() * sizeof(double));
+  if (e == hipSuccess) e = c.queue.ensure(256);
+  if (e != hipSuccess) {
+    set_error("hipMalloc (overlapped upload)", e);
+    return -2;
+  }
+  unsigned* flags = (unsigned*)c.gate.p;
+  unsigned* gerr = flags + K;
+  const hipStream_t st = c.stream;
+  hipLaunchKernelGGL(k_zero_u32, dim3((K + 1 + 255) / 256), dim3(256), 0, st, flags, K + 1);
+  hipLaunchKernelGGL(k_iota1, dim3((nCell + 255) / 256), dim3(256), 0, st, (int*)c.Perm.p, nCell);
+  if ((e = hipGetLastError()) == hipSuccess)
+    e = hipMemcpyAsync(c.ATOL.p, ATOL, (size_t)md.nvar * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c.RTOL.p, RTOL, (size_t)md.nvar * 8, hipMemcpyHostToDevice, st);
+  // the copies may start once the flags are clear; the kernel does not wait for them
+  if (e == hipSuccess) e = hipEventRecord(c.stage.done[0], st);
+  if (e != hipSuccess) {
+    set_error("overlapped upload set-up", e);
+    return -3;
+  }
+  GcBatchArgs b;
+  b.nCell = nCell;
+  b.Tstart = Tstart;
+  b.Tend = Tend;
+  b.C = (double*)c.C.p;
+  b.RCONST = (const double*)c.RCONST.p;
+  b.ATOL = (const double*)c.ATOL.p;
+  b.RTOL = (const double*)c.RTOL.p;
+  b.ICNTRL = (const int*)c.ICNTRL.p;
+  b.RCNTRL = (const double*)c.RCNTRL.p;
+  b.Perm = (const int*)c.Perm.p;
+  b.ISTAT = (int*)c.ISTAT.p;
+  b.RSTAT = (double*)c.RSTAT.p;
+  b.IERR = (int*)c.IERR.p;
+  const GcGate gate{flags, (unsigned)cells, g.spin, gerr};
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, (unsigned*)c.queue.p, 1);
+  if ((e = hipEventRecord(c.ev0, st)) == hipSuccess) {
+    const unsigned waves = S / 64u;
+    unsigned fair = ((unsigned)nCell + waves - 1u) / waves;
+    if (fair < 1u) fair = 1u;
+    const int grain = (unsigned)t.grain > fair ? (int)fair : t.grain;
+    if (t.variant == 2)
+      e = gc_launch_persistent_gated_v2(&b, (double*)c.ws.p, S, (unsigned*)c.queue.p, t.block,
+                                        t.cells_per_wave, grain, &gate, st);
+    else if (t.variant == 3)
+      e = gc_launch_persistent_gated_v3(&b, (double*)c.ws.p, S, (unsigned*)c.queue.p, t.block,
+                                        t.cells_per_wave, grain, &gate, st);
+    else
+      e = gc_launch_persistent_gated_v1(&b, (double*)c.ws.p, S, (unsigned*)c.queue.p, t.block,
+                                        t.cells_per_wave, grain, &gate, st);
+  }
+  if (e == hipSuccess) e = hipEventRecord(c.ev1, st);
+  // the inputs follow the launch, chunk by chunk, while it runs
+  const Piece pin[4] = {{c.C.p, const_cast<double*>(C), 0}, {c.RCONST.p, const_cast<double*>(RCONST), 0},
+                        {c.ICNTRL.p, const_cast<int*>(ICNTRL), 0},
+                        {c.RCNTRL.p, const_cast<double*>(RCNTRL), 0}};
+  hipError_t ue = hipSuccess;
+  if (e == hipSuccess)
+    ue = c.stage.to_device_gated(dev, pin, stride, 4, nCell, cells, flags, g.mode != 2,
+                                 c.stage.done[0]);
+  // the solve ends whatever the copies did: a chunk that never lands ends its
+  // waves' waits at the bound
+  hipError_t se = hipStreamSynchronize(st);
+  hipError_t ce = c.stage.sync();
+  if (e != hipSuccess || ue != hipSuccess || se != hipSuccess || ce != hipSuccess) {
+    set_error("overlapped upload / solve",
+              e != hipSuccess ? e : ue != hipSuccess ? ue : se != hipSuccess ? se : ce);
+    return -3;
+  }
+  unsigned bad = 0;
+  if ((e = hipMemcpy(&bad, gerr, sizeof(unsigned), hipMemcpyDeviceToHost)) != hipSuccess) {
+    set_error("overlapped upload: gate status", e);
+    return -3;
+  }
+  if (bad) {
+    set_error("overlapped upload: an input chunk did not land within the wait bound");
+    return -4;
+  }
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess) *tKernel = ms * 1e-3;
+  return 0;
 }
 
 // sort keys of the next launch order: the cell's internal step count of the
@@ -967,6 +1176,37 @@ void gc_gpu_integrate_batch_dev(int dev, int nCell, double Tstart, double Tend, 
   const MechDims md = mech_dims();
   const size_t n = (size_t)nCell;
   const size_t nspec = (size_t)(md.nvar + md.nfix);
+  const int kernel = env_kernel();
+  const GateConfig gate = current_gate();
+  bool identity = true;
+  for (int i = 0; i < nCell && identity; ++i) identity = Perm[i] == i + 1;
+  if (e == hipSuccess) c.stage.init();
+  if (e == hipSuccess && gate.mode != 0 && identity && kernel == GC_AMD_KERNEL_PERSISTENT &&
+      c.stage.ok) {
+    // the model's call (cells gathered in cost order, Perm = 1..nCell): the
+    // inputs stream in while the solve runs
+    const int rc = gated_solve(c, d, gate, nCell, Tstart, Tend, C, RCONST, ATOL, RTOL, ICNTRL,
+                               RCNTRL, tKernel);
+    if (rc == -4 && gate.mode == 1) {
+      // the copies did not run beside the kernel: no overlap in this process
+      std::lock_guard<std::mutex> glk(g_gate_mu);
+      g_gate.mode = 0;
+    } else if (rc != 0) {
+      fill_ierr(IERR, nCell, -98);
+      return;
+    } else {
+      Piece pout[4] = {{c.C.p, C, n * nspec * 8}, {c.ISTAT.p, ISTAT, n * 8 * 4},
+                       {c.RSTAT.p, RSTAT, n * 3 * 8}, {c.IERR.p, IERR, n * 4}};
+      e = c.stage.to_host(d, pout, 4, c.stream);
+      if (e != hipSuccess) {
+        set_error("copy back", e);
+        fill_ierr(IERR, nCell, -98);
+        return;
+      }
+      *tTotal = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      return;
+    }
+  }
   struct { DevBuf* b; const void* h; size_t bytes; } in[] = {
       {&c.C, C, n * nspec * 8}, {&c.RCONST, RCONST, n * md.nreact * 8},
       {&c.ATOL, ATOL, (size_t)md.nvar * 8}, {&c.RTOL, RTOL, (size_t)md.nvar * 8},
@@ -977,7 +1217,6 @@ void gc_gpu_integrate_batch_dev(int dev, int nCell, double Tstart, double Tend, 
     if (e == hipSuccess) e = x.b->ensure(x.bytes);
     pin[np++] = {x.b->p, const_cast<void*>(x.h), x.bytes};
   }
-  c.stage.init();
   if (e == hipSuccess) e = c.stage.to_device(d, pin, np, c.stream, c.stream);
   if (e == hipSuccess) e = c.ISTAT.ensure(n * 8 * 4);
   if (e == hipSuccess) e = c.RSTAT.ensure(n * 3 * 8);
@@ -1001,7 +1240,6 @@ void gc_gpu_integrate_batch_dev(int dev, int nCell, double Tstart, double Tend, 
   b.ISTAT = (int*)c.ISTAT.p;
   b.RSTAT = (double*)c.RSTAT.p;
   b.IERR = (int*)c.IERR.p;
-  const int kernel = env_kernel();
   const int rc = device_solve(c, kernel, kernel == GC_AMD_KERNEL_PLAIN ? env_chunk(nCell) : 0, b,
                               c.stream, tKernel);
   if (rc != 0) {
@@ -1414,6 +1652,22 @@ void gc_amd_get_tuning(int* block, int* blocks_per_cu, int* cells_per_wave, int*
   *cells_per_wave = t.cells_per_wave;
   *grain = t.grain;
   *variant = t.variant;
+}
+
+int gc_amd_set_gate(int mode, int cells, unsigned long long spin) {
+  if (mode < 0 || mode > 2 || cells < 0 || spin == 0ull) return -1;
+  std::lock_guard<std::mutex> lk(g_gate_mu);
+  g_gate.mode = mode;
+  g_gate.cells = cells;
+  g_gate.spin = spin;
+  return 0;
+}
+
+void gc_amd_get_gate(int* mode, int* cells, unsigned long long* spin) {
+  const GateConfig g = current_gate();
+  *mode = g.mode;
+  *cells = g.cells;
+  *spin = g.spin;
 }
 
 void gc_amd_mech_info(int* nvar, int* nfix, int* nreact, int* lu_nonzero) {

@@ -1074,17 +1074,38 @@ __device__ __forceinline__ void record_step(const Blk& b, const GcRecord& rec, b
   head = (int)slot;
 }
 
+// Waits until the input chunks of launch positions [q, q_end) have landed
+// (GcGate); seen = chunks 0..seen-1 already observed by this wave.  Returns
+// false when a wait ran past the gate's bound.  Wave-uniform.
+__device__ __forceinline__ bool gate_wait(const GcGate& g, unsigned q, unsigned q_end,
+                                          unsigned& seen) {
+  const unsigned k_end = (q_end - 1u) / g.cells + 1u;
+  if (k_end <= seen) return true;
+  for (unsigned k = seen; k < k_end; ++k) {
+    unsigned long long n = 0;
+    while (__builtin_amdgcn_readfirstlane(
+               __hip_atomic_load(g.flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) == 0u) {
+      if (++n > g.spin) return false;
+      __builtin_amdgcn_s_sleep(127);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the chunks' data is visible from here on
+  seen = k_end;
+  return true;
+}
+
 // The persistent grid over one batch `a`, or (MB) over nb batches described
 // by descs in one queue of qn positions: batch k's cells take positions
 // descs[k].q0 .. descs[k].q0 + descs[k].a.nCell - 1 in its own launch order,
 // each lane working with its current cell's batch arguments.  A cell's record
-// chain head (recording form) is head[q0 + cell].
-template <bool REC, bool M = false, bool MB = false>
+// chain head (recording form) is head[q0 + cell].  GATE: positions are handed
+// out only once their input chunk has landed (GcGate).
+template <bool REC, bool M = false, bool MB = false, bool GATE = false>
 __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws, unsigned S,
                                                 unsigned* queue, int cells_per_wave, int grain,
                                                 const GcRecord& rec,
                                                 const GcBatchDesc* descs = nullptr, int nb = 0,
-                                                unsigned qn = 0) {
+                                                unsigned qn = 0, const GcGate& gate = GcGate{}) {
   extern __shared__ double lds[];
   const unsigned slot = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -1106,6 +1127,7 @@ __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws
   unsigned q0 = 0;                // its first queue position
   unsigned* ldone = nullptr;      // its finished-cell counter (MB, may be null)
   int bi = 0;                     // its index (MB)
+  unsigned gate_seen = 0;         // input chunks this wave has seen land (GATE)
   const unsigned qtot = MB ? qn : (unsigned)a.nCell;
   // a cell's results are out: count it for the caller's progress report
   auto finished = [&]() {
@@ -1129,6 +1151,14 @@ __device__ __forceinline__ void persistent_body(const GcBatchArgs& a, double* ws
         base = __shfl(base, leader);
         rsv = base;
         rsv_end = base + take;
+        if (GATE && rsv < qtot && !gate_wait(gate, rsv, rsv_end < qtot ? rsv_end : qtot,
+                                             gate_seen)) {
+          // an input chunk never landed: drop the reservation, take no more cells
+          if (lane == leader)
+            __hip_atomic_fetch_or(gate.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          rsv = rsv_end;
+          exhausted = true;
+        }
       }
       const unsigned avail = rsv_end - rsv;
       const unsigned rank = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
@@ -1265,6 +1295,15 @@ __attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE,
 GC_SFX(gc_rodas3_persistent)(GcBatchArgs a, double* ws, unsigned S, unsigned* queue,
                              int cells_per_wave, int grain) {
   persistent_body<false>(a, ws, S, queue, cells_per_wave, grain, GcRecord{});
+}
+
+// The persistent kernel behind the input gate of the overlapped upload.
+extern "C" __global__ void
+__attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(GC_WPE, GC_WPE)))
+GC_SFX(gc_rodas3_persistent_gated)(GcBatchArgs a, double* ws, unsigned S, unsigned* queue,
+                                   int cells_per_wave, int grain, GcGate gate) {
+  persistent_body<false, false, false, true>(a, ws, S, queue, cells_per_wave, grain, GcRecord{},
+                                             nullptr, 0, 0, gate);
 }
 
 // One persistent launch over nb batches (their own arrays, sizes, launch
@@ -1755,6 +1794,17 @@ extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_persistent)(const GcBatchArgs
   else
     hipLaunchKernelGGL(GC_SFX(gc_rodas3_persistent), dim3(grid), dim3(block), lds, st, *a, ws, S,
                        queue, cells_per_wave, grain);
+  return hipGetLastError();
+}
+
+extern "C" GC_INTERNAL hipError_t GC_SFX(gc_launch_persistent_gated)(
+    const GcBatchArgs* a, double* ws, unsigned S, unsigned* queue, int block, int cells_per_wave,
+    int grain, const GcGate* gate, hipStream_t st) {
+  if (a->nCell <= 0) return hipSuccess;
+  const unsigned grid = S / (unsigned)block;
+  const size_t lds = (size_t)(block / 64) * (GC_SEG + 1) * 64 * sizeof(double);
+  hipLaunchKernelGGL(GC_SFX(gc_rodas3_persistent_gated), dim3(grid), dim3(block), lds, st, *a, ws,
+                     S, queue, cells_per_wave, grain, *gate);
   return hipGetLastError();
 }
 

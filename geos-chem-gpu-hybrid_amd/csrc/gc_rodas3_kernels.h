@@ -35,6 +35,20 @@ struct GcBatchDesc {
   unsigned* done;  // cells of the batch finished so far (host-visible), or null
 };
 
+// Input gate of the overlapped upload of the host-array entry point: the
+// batch's inputs arrive in HBM in chunks of `cells` consecutive launch
+// positions while the persistent kernel already runs; flags[k] turns nonzero
+// (written by the copy engine after chunk k's data) once chunk k has landed.
+// A wave hands out a position only once its chunk's flag is seen, waiting at
+// most `spin` sleeps of ~3.4 us per chunk; past that it sets *err and takes
+// no more cells (every wait ends).
+struct GcGate {
+  const unsigned* flags;
+  unsigned cells;
+  unsigned long long spin;
+  unsigned* err;
+};
+
 // What a compiled variant is: mechanism dimensions, doubles of workspace per
 // cell slot (the workspace is slots/64 wave blocks of 64*ws_doubles_per_slot
 // doubles), its occupancy target and its LU row-segment size in LDS.
@@ -108,6 +122,9 @@ struct GcAdjDesc {
                                                    unsigned S, unsigned* queue,       \
                                                    int block, int cells_per_wave,     \
                                                    int grain, hipStream_t st);        \
+  extern "C" GC_INTERNAL hipError_t gc_launch_persistent_gated_v##V(                  \
+      const GcBatchArgs* a, double* ws, unsigned S, unsigned* queue, int block,       \
+      int cells_per_wave, int grain, const GcGate* gate, hipStream_t st);             \
   extern "C" GC_INTERNAL hipError_t gc_launch_record_v##V(                            \
       const GcBatchArgs* a, double* ws, unsigned S, unsigned* queue, int block,       \
       int cells_per_wave, int grain, const GcRecord* rec, hipStream_t st);            \

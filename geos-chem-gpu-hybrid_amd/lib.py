@@ -90,6 +90,8 @@ SIGNATURES = {
                                                        CPU_TAIL_FN, P]),
     "gc_amd_set_tuning": (I, [I, I, I, I, I]),
     "gc_amd_get_tuning": (None, [IP, IP, IP, IP, IP]),
+    "gc_amd_set_gate": (I, [I, I, ctypes.c_ulonglong]),
+    "gc_amd_get_gate": (None, [IP, IP, ctypes.POINTER(ctypes.c_ulonglong)]),
     "gc_amd_mech_info": (None, [IP, IP, IP, IP]),
     "gc_amd_workspace_doubles_per_slot": (ctypes.c_long, []),
     "gc_amd_last_error": (ctypes.c_char_p, []),
@@ -156,3 +158,17 @@ def set_tuning(lib, block=0, blocks_per_cu=-1, cells_per_wave=0, grain=0, varian
     if rc != 0:
         raise ValueError(f"gc_amd_set_tuning rejected "
                          f"{block, blocks_per_cu, cells_per_wave, grain, variant}")
+
+
+def get_gate(lib) -> dict:
+    m, c, sp = ctypes.c_int(), ctypes.c_int(), ctypes.c_ulonglong()
+    lib.gc_amd_get_gate(ctypes.byref(m), ctypes.byref(c), ctypes.byref(sp))
+    return {"mode": m.value, "cells": c.value, "spin": sp.value}
+
+
+def set_gate(lib, mode=1, cells=0, spin=600000):
+    """Overlapped upload of the host-array entry points (gc_amd_set_gate):
+    mode 1 on, 0 off, 2 the failure path (chunks never signalled)."""
+    rc = lib.gc_amd_set_gate(mode, cells, spin)
+    if rc != 0:
+        raise ValueError(f"gc_amd_set_gate rejected {mode, cells, spin}")

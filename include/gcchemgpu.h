@@ -291,6 +291,26 @@ int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave,
 void gc_amd_get_tuning(int *block, int *blocks_per_cu, int *cells_per_wave,
                        int *grain, int *variant);
 
+/* Overlapped upload of the host-array entry points (no counterpart in the
+ * reference, whose gc_gpu_batch_api.F90:199-211 stages the whole batch before
+ * the launch).  With mode 1 (the default) a gc_gpu_integrate_batch[_dev] call
+ * whose launch order is the identity -- the model gathers its batch in cost
+ * order and passes Perm = 1..nCell (fullchem_mod.F90:2359-2360) -- and that
+ * runs the persistent kernel starts the solve first and uploads C, RCONST,
+ * ICNTRL and RCNTRL in chunks of `cells` launch positions (0: what fits one
+ * 32 MB staging buffer) while it runs; a wave hands out a position only once
+ * its chunk has landed, waiting at most `spin` sleeps of ~3.4 us per chunk
+ * (default 600,000, ~2 s; the 4x5 grid's whole upload takes ~70 ms).  A call
+ * whose chunks do not land within that bound (copies that cannot run beside
+ * the kernel) switches the overlap off for the rest of the process (mode 0)
+ * and is solved again with the upload first.  Mode 0 uploads everything
+ * first.  Mode 2 is mode 1 with the chunks' arrival never signalled, so every
+ * wait runs to its bound and the call fails with IERR -98 for every cell:
+ * the failure path, for tests.  Results are identical in modes 0 and 1.
+ * Returns 0, or -1 for a value out of range (nothing changed). */
+int gc_amd_set_gate(int mode, int cells, unsigned long long spin);
+void gc_amd_get_gate(int *mode, int *cells, unsigned long long *spin);
+
 /* ---- discrete adjoint (frozen step sizes) -------------------------------- */
 
 /* Forward batch solve (persistent kernel, same arguments and results as

@@ -213,6 +213,67 @@ def test_host_array_entry_points(cells2k, ref2k, mech):
     assert tt.value == 0.0 and tk.value == 0.0
 
 
+def _host_call(lib, d, perm):
+    """gc_gpu_integrate_batch on host copies of d's arrays; returns the outputs."""
+    n = d["C"].shape[0]
+    C = np.ascontiguousarray(d["C"].copy())
+    IS = np.zeros((n, 8), np.int32)
+    RS = np.zeros((n, 3))
+    IE = np.zeros(n, np.int32)
+    tt, tk = ctypes.c_double(), ctypes.c_double()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib.gc_gpu_integrate_batch(n, 0.0, 600.0, p(C), p(d["RCONST"]), p(d["ATOL"]), p(d["RTOL"]),
+                               p(d["ICNTRL"]), p(d["RCNTRL"]), p(perm), p(IS), p(RS), p(IE),
+                               ctypes.byref(tt), ctypes.byref(tk))
+    return (C, IS, RS, IE), tt.value, tk.value
+
+
+@pytest.mark.parametrize("cells", [64, 0])
+def test_overlapped_upload_in_launch_order(cells2k, ref2k, mech, cells):
+    """The model's call: the batch gathered in cost order with Perm = 1..nCell
+    (fullchem_mod.F90:2359-2360).  The inputs then stream in while the solve
+    runs, chunk by chunk behind the kernel's input gate (64-cell chunks: 32
+    chunks; 0: one staging slot's worth, the whole batch); results equal the
+    oracle and the upload-first path bit for bit."""
+    lib = L.load()
+    n = cells2k["C"].shape[0]
+    order = np.random.default_rng(5).permutation(n)  # any gather order of the caller's cells
+    d = inputs.subset(cells2k, order)
+    ident = np.arange(1, n + 1, dtype=np.int32)
+    ref = tuple(x[order] for x in ref2k[:4])
+    try:
+        L.set_gate(lib, mode=1, cells=cells)
+        out, tt, tk = _host_call(lib, d, ident)
+        assert tt > 0 and tk > 0, L.last_error(lib)
+        _assert_bitexact(out, ref, mech.nvar)
+        L.set_gate(lib, mode=0)
+        out0, _, _ = _host_call(lib, d, ident)
+        for a, b in zip(out, out0):
+            assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+    finally:
+        L.set_gate(lib, mode=1, cells=0)
+
+
+def test_overlapped_upload_failure_path(cells2k, ref2k, mech):
+    """A chunk whose arrival is never signalled (mode 2) ends every wave's wait
+    at the bound (100 sleeps, ~0.3 ms): the call returns IERR -98 for every
+    cell, for the caller's retry, and the next call is served normally."""
+    lib = L.load()
+    n = 256
+    d = inputs.subset(cells2k, np.arange(n))
+    ident = np.arange(1, n + 1, dtype=np.int32)
+    try:
+        L.set_gate(lib, mode=2, cells=64, spin=100)
+        out, _, _ = _host_call(lib, d, ident)
+        assert np.all(out[3] == -98)
+        assert "did not land" in L.last_error(lib)
+        L.set_gate(lib, mode=1, cells=64)
+        out, _, _ = _host_call(lib, d, ident)
+        _assert_bitexact(out, tuple(x[:n] for x in ref2k[:4]), mech.nvar)
+    finally:
+        L.set_gate(lib, mode=1, cells=0)
+
+
 def test_environment_selects_kernel_and_chunks(cells2k, ref2k, mech, monkeypatch):
     """The reference's run-time switches (gpulib/src/gc_gpu_batch_api.F90:
     231-256), read at every host-array call: GC_GPU_KERNEL=plain with
