@@ -1384,6 +1384,7 @@ __device__ void adj_nu(const Blk& b) {
     double x[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) x[r] = b.ld(tt[r].x);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int r = 0; r < 16; ++r) stream_term(b, tt[r].y, dbl(tt[r].z, tt[r].w) * x[r], acc);
   }
@@ -1394,6 +1395,7 @@ __device__ void adj_nu(const Blk& b) {
     double x[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) x[r] = b.ld(tt[r].x);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int r = 0; r < 8; ++r) stream_term(b, tt[r].y, dbl(tt[r].z, tt[r].w) * x[r], acc);
   }
@@ -1414,14 +1416,33 @@ __device__ __forceinline__ int eoff(int w, int sh) { return ((w >> sh) & 0xFFFF)
 
 // second order, records of element indices {GK | CN<<16, x1 | x2<<16,
 // x3 | d1<<16, d2 | d3<<16} of NF factors, CN = c*nu: GK += CN*(sum_g prod_{h!=g} x_h d_g)
+struct Gk2Ops {
+  double g, cn, d1, x1, x2, d2, x3, d3;
+};
+
 template <int NF>
-__device__ __forceinline__ double gk2_term(const Blk& b, i4 f) {
-  const double d1 = b.ld(eoff(f.z, 16));
-  if (NF == 1) return d1;
-  const double x1 = b.ld(eoff(f.y, 0)), x2 = b.ld(eoff(f.y, 16)), d2 = b.ld(eoff(f.w, 0));
-  if (NF == 2) return (x2 * d1) + (x1 * d2);
-  const double x3 = b.ld(eoff(f.z, 0)), d3 = b.ld(eoff(f.w, 16));
-  return (((x2 * x3) * d1) + ((x1 * x3) * d2)) + ((x1 * x2) * d3);
+__device__ __forceinline__ Gk2Ops gk2_load(const Blk& b, i4 f) {
+  Gk2Ops o;
+  o.g = b.ld(eoff(f.x, 0));
+  o.cn = b.ld(eoff(f.x, 16));
+  o.d1 = b.ld(eoff(f.z, 16));
+  if (NF >= 2) {
+    o.x1 = b.ld(eoff(f.y, 0));
+    o.x2 = b.ld(eoff(f.y, 16));
+    o.d2 = b.ld(eoff(f.w, 0));
+  }
+  if (NF >= 3) {
+    o.x3 = b.ld(eoff(f.z, 0));
+    o.d3 = b.ld(eoff(f.w, 16));
+  }
+  return o;
+}
+
+template <int NF>
+__device__ __forceinline__ double gk2_term(const Gk2Ops& o) {
+  if (NF == 1) return o.d1;
+  if (NF == 2) return (o.x2 * o.d1) + (o.x1 * o.d2);
+  return (((o.x2 * o.x3) * o.d1) + ((o.x1 * o.x3) * o.d2)) + ((o.x1 * o.x2) * o.d3);
 }
 
 template <int NF>
@@ -1430,15 +1451,12 @@ __device__ __forceinline__ ctab_t* gk2_group(const Blk& b, ctab_t* q, int n) {
     i4 f[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) f[u] = ld4(q + 4 * u);
-    double g[8], cn[8], p[8];
+    Gk2Ops o[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      g[u] = b.ld(eoff(f[u].x, 0));
-      cn[u] = b.ld(eoff(f[u].x, 16));
-      p[u] = gk2_term<NF>(b, f[u]);
-    }
+    for (int u = 0; u < 8; ++u) o[u] = gk2_load<NF>(b, f[u]);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) b.st(eoff(f[u].x, 0), g[u] + cn[u] * p[u]);
+    for (int u = 0; u < 8; ++u) b.st(eoff(f[u].x, 0), o[u].g + o[u].cn * gk2_term<NF>(o[u]));
   }
   return q;
 }
@@ -1471,6 +1489,7 @@ __device__ void adj_lp(const Blk& b, const int* table, int nterm) {
       xc[r] = b.ld(eoff(tt[r].y, 16));
       xd[r] = b.ld(eoff(tt[r].z, 0));
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int r = 0; r < 16; ++r)
       stream_term(b, tt[r].w, w[r] * ((xa[r] * xb[r]) + (xc[r] * xd[r])), acc);
@@ -1488,6 +1507,7 @@ __device__ void adj_lp(const Blk& b, const int* table, int nterm) {
       xc[r] = b.ld(eoff(tt[r].y, 16));
       xd[r] = b.ld(eoff(tt[r].z, 0));
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int r = 0; r < 8; ++r)
       stream_term(b, tt[r].w, w[r] * ((xa[r] * xb[r]) + (xc[r] * xd[r])), acc);
@@ -1503,7 +1523,7 @@ __device__ void adj_wk(const Blk& b, const int* table) {
     i4 tt[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) tt[r] = ld4(t + 4 * r);
-    double g[8], nu[8], k[8], x1[8], x2[8], x3[8];
+    double g[8], nu[8], k[8], x1[8], x2[8], x3[8], lit[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       g[r] = b.ld(eoff(tt[r].x, 16));
@@ -1512,10 +1532,12 @@ __device__ void adj_wk(const Blk& b, const int* table) {
       x1[r] = b.ld(eoff(tt[r].z, 0));
       x2[r] = b.ld(eoff(tt[r].z, 16));
       x3[r] = b.ld(eoff(tt[r].w, 0));
+      lit[r] = ((clit_t*)gc_lits)[(tt[r].w >> 16) & 0xFFFF];
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      const double cn = ((clit_t*)gc_lits)[(tt[r].w >> 16) & 0xFFFF] * nu[r];
+      const double cn = lit[r] * nu[r];
       b.st(eoff(tt[r].x, 0), cn * k[r]);
       b.st(eoff(tt[r].x, 16), g[r] + cn * ((x1[r] * x2[r]) * x3[r]));
       b.st(eoff(tt[r].y, 0), cn);  // c*nu for the second-order records
