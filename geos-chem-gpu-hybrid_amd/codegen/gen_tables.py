@@ -34,7 +34,9 @@ Tables (int32 words):
   ``prodgrp_<name>`` gives the record count of each of the 8 shapes
   (shape = (nf-1)*2 + has_lit).  Value = lit*f0*f1*... left to right.
   Products: ``rate_y``/``rate_yn`` (A(r), V = Y or Ynew; gckpp_Function.F90:
-  2215-3248), ``bjac`` (Jac_SP's B; gckpp_Jacobian.F90:63-3673).
+  2215-3248), ``bjac`` (Jac_SP's B; gckpp_Jacobian.F90:63-3673; a B that is a rate
+  constant alone is not formed: its JVS terms read RCONST directly, see
+  Tables.b_alias).
 * ``pd_y``/``pd_yn``: per species 4-word header ``{nP, nD, V(i), i*512}``
   then its 4-word P terms ``{soff, 0, coef_lo, coef_hi}`` (P_VAR(i) over A)
   and D product terms ``{f1, f2, f3, ci}`` (D_VAR(i), each term
@@ -214,9 +216,20 @@ class Tables:
         return self.product_group([(f, vbase, self.off("A", r))
                                    for r, f in enumerate(self.ir["rate"])])
 
+    def b_alias(self, b):
+        """The RCT element B(b) equals when Jac_SP's B(b) is a rate constant
+        alone (B(b) = RCONST(r), gckpp_Jacobian.F90:63-3673), else None: the
+        JVS terms then read RCONST(r) itself, bit for bit the same value, and
+        B(b) is neither computed nor stored."""
+        f = self.ir["jac_b"][b]
+        if f is not None and len(f) == 1 and f[0][0] == "RCT":
+            return self.off("RCT", f[0][1])
+        return None
+
     def bjac(self):
         return self.product_group([(f, "Y", self.off("B", b))
-                                   for b, f in enumerate(self.ir["jac_b"]) if f is not None])
+                                   for b, f in enumerate(self.ir["jac_b"])
+                                   if f is not None and self.b_alias(b) is None])
 
     # -- sums ------------------------------------------------------------------
     def sum_term(self, sign, prod, resolve) -> list[int]:
@@ -272,7 +285,10 @@ class Tables:
         slot."""
         if self.ir["nnz"] >= 1 << 18:
             raise ValueError("JVS index beyond the 18 bits of an entry term's flags")
-        resolve = lambda x: self.data_off(x, "Y")
+        def resolve(x):
+            if x[0] == "B" and self.b_alias(x[1]) is not None:
+                return self.b_alias(x[1])
+            return self.data_off(x, "Y")
         zero = self.off("ZERO", 0)
         words = []
         for sl, dg, terms, k in entries:
