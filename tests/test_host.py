@@ -94,3 +94,20 @@ def test_solver_rejects_host_tensors_and_bad_shapes(mech):
     b.C = torch.zeros((4, mech.nspec + 1), dtype=torch.float64)
     with pytest.raises((TypeError, ValueError)):
         solver._check(b, (mech.nvar, mech.nfix, mech.nreact, mech.nnz))
+
+
+def test_gate_settings_round_trip_and_reject_bad_values():
+    """gc_amd_set_gate / gc_amd_get_gate (the overlapped upload of the
+    host-array entry points): values round-trip, out-of-range values are
+    refused and change nothing.  No device is touched."""
+    from gcchem_amd import lib as L
+    lib = L.load()
+    before = L.get_gate(lib)
+    try:
+        L.set_gate(lib, mode=0, cells=128, spin=1000)
+        assert L.get_gate(lib) == {"mode": 0, "cells": 128, "spin": 1000}
+        for bad in ((3, 0, 1), (-1, 0, 1), (1, -64, 1), (1, 0, 0)):
+            assert lib.gc_amd_set_gate(*bad) == -1
+        assert L.get_gate(lib) == {"mode": 0, "cells": 128, "spin": 1000}
+    finally:
+        L.set_gate(lib, **before)
