@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <strings.h>
 
 #include <algorithm>
 #include <chrono>
@@ -106,9 +107,24 @@ struct GateConfig {
 };
 std::mutex g_gate_mu;
 GateConfig g_gate;
+bool g_gate_env_read = false;
+
+bool env_set(const char* name) {
+  const char* s = getenv(name);
+  return s && *s && strcmp(s, "0") != 0 && strcasecmp(s, "false") != 0;
+}
 
 GateConfig current_gate() {
   std::lock_guard<std::mutex> lk(g_gate_mu);
+  if (!g_gate_env_read) {
+    g_gate_env_read = true;
+    // Where the device work is serialized around each dispatch (rocprofv3
+    // counter collection, the launch-blocking debug modes), the copies cannot
+    // land while the gated kernel runs: the overlap starts off there.
+    for (const char* v : {"ROCPROF_COUNTER_COLLECTION", "HIP_LAUNCH_BLOCKING",
+                          "AMD_SERIALIZE_KERNEL", "AMD_SERIALIZE_COPY"})
+      if (env_set(v)) g_gate.mode = 0;
+  }
   return g_gate;
 }
 
@@ -1655,6 +1671,7 @@ void gc_amd_get_tuning(int* block, int* blocks_per_cu, int* cells_per_wave, int*
 }
 
 int gc_amd_set_gate(int mode, int cells, unsigned long long spin) {
+  (void)current_gate();  // read the environment first so explicit values win
   if (mode < 0 || mode > 2 || cells < 0 || spin == 0ull) return -1;
   std::lock_guard<std::mutex> lk(g_gate_mu);
   g_gate.mode = mode;

@@ -307,6 +307,10 @@ void gc_amd_get_tuning(int *block, int *blocks_per_cu, int *cells_per_wave,
  * first.  Mode 2 is mode 1 with the chunks' arrival never signalled, so every
  * wait runs to its bound and the call fails with IERR -98 for every cell:
  * the failure path, for tests.  Results are identical in modes 0 and 1.
+ * The default is mode 0 when the process starts under rocprofv3 counter
+ * collection (ROCPROF_COUNTER_COLLECTION) or a launch-serializing debug mode
+ * (HIP_LAUNCH_BLOCKING, AMD_SERIALIZE_KERNEL, AMD_SERIALIZE_COPY), where the
+ * copies could not land while the kernel runs.
  * Returns 0, or -1 for a value out of range (nothing changed). */
 int gc_amd_set_gate(int mode, int cells, unsigned long long spin);
 void gc_amd_get_gate(int *mode, int *cells, unsigned long long *spin);

@@ -111,3 +111,24 @@ def test_gate_settings_round_trip_and_reject_bad_values():
         assert L.get_gate(lib) == {"mode": 0, "cells": 128, "spin": 1000}
     finally:
         L.set_gate(lib, **before)
+
+
+def test_gate_starts_off_under_serializing_tools():
+    """Under rocprofv3 counter collection or a launch-serializing debug mode
+    the overlapped upload starts off (mode 0): the copies could not land while
+    the gated kernel runs.  A process without them starts with it on."""
+    import os
+    import subprocess
+    import sys
+    from conftest import REPO
+    code = ("import sys; sys.path.insert(0, %r); import __graft_entry__ as g; g.load_package(); "
+            "from gcchem_amd import lib as L; print(L.get_gate(L.load())['mode'])" % str(REPO))
+    env = {k: v for k, v in os.environ.items()
+           if k not in ("ROCPROF_COUNTER_COLLECTION", "HIP_LAUNCH_BLOCKING",
+                        "AMD_SERIALIZE_KERNEL", "AMD_SERIALIZE_COPY")}
+    run = lambda e: subprocess.run([sys.executable, "-c", code], env=e, capture_output=True,
+                                   text=True, timeout=300, check=True).stdout.split()[-1]
+    assert run(env) == "1"
+    assert run({**env, "ROCPROF_COUNTER_COLLECTION": "True"}) == "0"
+    assert run({**env, "AMD_SERIALIZE_KERNEL": "3"}) == "0"
+    assert run({**env, "HIP_LAUNCH_BLOCKING": "0"}) == "1"
