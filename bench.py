@@ -184,6 +184,47 @@ def host_api_rate(lib, d, perm):
     return n / tT.value, tT.value, tK.value
 
 
+def gate_sweep(lib, d, perm, b, sizes, reps=2):
+    """The host-array entry point in launch order at several input-gate chunk
+    sizes (gc_amd_set_gate; cells per chunk, 0 = the library's default, -1 =
+    upload first), each `reps` times, beside the same solve with its inputs
+    already in HBM: one JSON record per case.  The gate setting is restored
+    afterwards."""
+    from gcchem_amd import lib as L, solver
+    import torch
+    n = d["C"].shape[0]
+    out = None
+    tk = []
+    for _ in range(3):
+        out = solver.solve(b, kernel="persistent", out=out)
+        torch.cuda.synchronize()
+        tk.append(out.t_kernel)
+    ref = float(min(tk))
+    log(f"inputs in HBM: kernel {', '.join(f'{t * 1e3:.1f}' for t in tk)} ms")
+    host_api_rate(lib, d, perm)  # pinned staging and device buffers set up, untimed
+    saved = L.get_gate(lib)
+    rows = []
+    try:
+        for cells in sizes:
+            if cells < 0:
+                L.set_gate(lib, 0, 0, saved["spin"])
+            else:
+                L.set_gate(lib, 1, cells, saved["spin"])
+            for _ in range(reps):
+                _, tt, tkk = host_api_rate(lib, d, perm)
+                rows.append({"gate_cells": cells, "t_total_ms": round(tt * 1e3, 1),
+                             "t_kernel_ms": round(tkk * 1e3, 1),
+                             "kernel_over_resident_ms": round((tkk - ref) * 1e3, 1),
+                             "outside_kernel_ms": round((tt - tkk) * 1e3, 1),
+                             "frac_of_resident": round(ref / tt, 4)})
+                log(f"gate cells {cells}: tTotal {tt * 1e3:.1f} ms, tKernel {tkk * 1e3:.1f} ms, "
+                    f"frac of the in-HBM kernel {ref / tt:.4f}")
+    finally:
+        L.set_gate(lib, saved["mode"], saved["cells"], saved["spin"])
+    return {"workload": f"fullchem 4x5 grid, {n} cells, launch order, host arrays",
+            "resident_kernel_ms": round(ref * 1e3, 1), "cases": rows}
+
+
 def resident_rate(d):
     """Two chemistry steps of a device-resident batch (gc_amd_resident_*):
     the first stands for the previous step (identity order, untimed), the
@@ -399,6 +440,10 @@ def main():
                     help="stress legs: fwd (always), plain, adj")
     ap.add_argument("--stress-max-steps", type=int, default=1000,
                     help="ICNTRL(4) of the stress cells (0: the integrator's 200,000)")
+    ap.add_argument("--gate-sweep", default="",
+                    help="comma-separated input-gate chunk sizes in cells (0 = default, "
+                         "-1 = upload first): time the host-array entry point at each and "
+                         "print one record instead of the bench line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB elementwise copy first (counter calibration)")
     args = ap.parse_args()
@@ -439,6 +484,10 @@ def main():
     perm = inputs.cost_order(r.ISTAT[:, 2].cpu().numpy())
     b.Perm = torch.as_tensor(perm, device=b.C.device)
     out = None
+    if args.gate_sweep:
+        sizes = [int(x) for x in args.gate_sweep.split(",") if x.strip()]
+        print(json.dumps(gate_sweep(lib, d, perm, b, sizes)), flush=True)
+        return
 
     def timed(kernel, steps, warmup):
         nonlocal out
