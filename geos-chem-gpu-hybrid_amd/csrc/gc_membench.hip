@@ -9,7 +9,10 @@
 //   stream   plain streaming read of the whole buffer (all waves, grid-stride)
 //   gather   random 512-byte wave reads, `loads_in_flight` (4, 8, 16 or 32)
 //            in flight per wave, with one 512-byte wave write per
-//            `loads_per_store` reads
+//            `loads_per_store` reads; each batch of reads (and the writes
+//            after it) falls in a window of `window` consecutive elements at
+//            a random place in the block (window = block: fully random), so
+//            small windows measure what address locality within a batch buys
 //   chain    latency of one 512-byte wave read whose address depends on the
 //            previous read (one wave per CU, each in its own block): the
 //            round trip a chain of dependent operations pays per link
@@ -17,7 +20,7 @@
 // of the gather is CUs * waves_per_cu * block_doubles * 512 bytes, so small
 // blocks measure the shape out of the on-chip caches.  It prints one JSON
 // line.  Usage:
-//   gc_membench [block_doubles waves_per_cu loads_per_store loads_in_flight]
+//   gc_membench [block_doubles waves_per_cu loads_per_store loads_in_flight window]
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
@@ -27,20 +30,25 @@ namespace {
 
 template <int kLoads>
 __global__ void __launch_bounds__(256) k_gather(double* buf, long block_elems, int iters,
-                                               int loads_per_store, double* out) {
+                                               int loads_per_store, unsigned window,
+                                               double* out) {
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   double* base = buf + wave * block_elems * 64;
   unsigned s = 12345u + (unsigned)wave * 7919u;
   const unsigned nel = (unsigned)block_elems;
+  const unsigned places = nel - window + 1u;  // 1 <= window <= nel (checked on the host)
+  unsigned w0 = 0;
   double acc = 0.0;
   int since = 0;
   for (int it = 0; it < iters; ++it) {
     double v[kLoads];
+    s = s * 1664525u + 1013904223u;
+    w0 = __builtin_amdgcn_readfirstlane((s >> 8) % places);
 #pragma unroll
     for (int q = 0; q < kLoads; ++q) {
       s = s * 1664525u + 1013904223u;
-      const unsigned e = __builtin_amdgcn_readfirstlane((s >> 8) % nel);
+      const unsigned e = __builtin_amdgcn_readfirstlane(w0 + (s >> 8) % window);  // < nel
       v[q] = base[(long)e * 64 + lane];
     }
 #pragma unroll
@@ -49,7 +57,7 @@ __global__ void __launch_bounds__(256) k_gather(double* buf, long block_elems, i
     while (loads_per_store > 0 && since >= loads_per_store) {
       since -= loads_per_store;
       s = s * 1664525u + 1013904223u;
-      const unsigned e = __builtin_amdgcn_readfirstlane((s >> 8) % nel);
+      const unsigned e = __builtin_amdgcn_readfirstlane(w0 + (s >> 8) % window);
       base[(long)e * 64 + lane] = acc;
     }
   }
@@ -97,8 +105,9 @@ int main(int argc, char** argv) {
   int wpc = argc > 2 ? atoi(argv[2]) : 8;
   int lps = argc > 3 ? atoi(argv[3]) : 5;
   int kl = argc > 4 ? atoi(argv[4]) : 16;
+  long window = argc > 5 ? atol(argv[5]) : block;
   if (block < 64 || block > 65536 || wpc < 4 || wpc > 32 || wpc % 4 || lps < 0 ||
-      (kl != 4 && kl != 8 && kl != 16 && kl != 32)) {
+      (kl != 4 && kl != 8 && kl != 16 && kl != 32) || window < 1 || window > block) {
     fprintf(stderr, "gc_membench: bad arguments\n");
     return 2;
   }
@@ -130,10 +139,10 @@ int main(int argc, char** argv) {
   const int grid = (int)(waves * 64 / 256);
   auto gather = [&](int it) {
     switch (kl) {
-      case 4: hipLaunchKernelGGL(k_gather<4>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, out); break;
-      case 8: hipLaunchKernelGGL(k_gather<8>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, out); break;
-      case 16: hipLaunchKernelGGL(k_gather<16>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, out); break;
-      default: hipLaunchKernelGGL(k_gather<32>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, out); break;
+      case 4: hipLaunchKernelGGL(k_gather<4>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, (unsigned)window, out); break;
+      case 8: hipLaunchKernelGGL(k_gather<8>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, (unsigned)window, out); break;
+      case 16: hipLaunchKernelGGL(k_gather<16>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, (unsigned)window, out); break;
+      default: hipLaunchKernelGGL(k_gather<32>, dim3(grid), dim3(256), 0, 0, buf, block, it, lps, (unsigned)window, out); break;
     }
   };
   gather(1);
@@ -156,10 +165,10 @@ int main(int argc, char** argv) {
   hipEventElapsedTime(&ms, a, b);
   const double chain_ns = ms * 1e6 / links;
   printf("{\"device\": \"%s\", \"cus\": %d, \"block_doubles\": %ld, \"waves_per_cu\": %d, "
-         "\"loads_per_store\": %d, \"loads_in_flight\": %d, \"buffer_gb\": %.2f, "
+         "\"loads_per_store\": %d, \"loads_in_flight\": %d, \"window\": %ld, \"buffer_gb\": %.2f, "
          "\"stream_read_tbs\": %.3f, "
          "\"gather_tbs\": %.3f, \"chain_ns\": %.1f, \"chain_set_mb\": %.1f}\n",
-         prop.gcnArchName, cus, block, wpc, lps, kl, n * 8.0 / 1e9, stream_tbs, gather_tbs, chain_ns,
+         prop.gcnArchName, cus, block, wpc, lps, kl, window, n * 8.0 / 1e9, stream_tbs, gather_tbs, chain_ns,
          cus * block * 512.0 / 1e6);
   hipFree(buf);
   hipFree(out);
