@@ -304,6 +304,15 @@ def bench_grid_2x25(args, mech, lib, tune):
         if k >= args.warmup:
             tk.append(out.t_kernel)
     ok = float((out.IERR == 1).float().mean().item())
+    # the same solve as one persistent launch (its workspace is one slot per
+    # resident lane whatever the cell count, so it needs no chunks)
+    c_chunked = out.C.clone()
+    ru = solver.solve(b, kernel="persistent", out=out)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(ru.C, c_chunked))
+    del c_chunked
+    log(f"persistent, one launch: {ru.t_kernel * 1e3:.1f} ms (C identical to the chunked solve: {same})")
+    t_one = ru.t_kernel
     rp = solver.solve(b, kernel="plain", out=out)
     torch.cuda.synchronize()
     log(f"plain: {rp.t_kernel * 1e3:.1f} ms")
@@ -319,6 +328,8 @@ def bench_grid_2x25(args, mech, lib, tune):
                                "as one HIP graph", "cells": n, "chunk": chunk, "tuning": tune},
         "plain_cells_per_s": round(n / rp.t_kernel, 1),
         "speedup_vs_plain": round(rp.t_kernel / t, 3),
+        "persistent_one_launch_cells_per_s": round(n / t_one, 1),
+        "one_launch_equals_chunked": same,
         "ierr_ok_frac": ok,
         "roofline": roofline(mech, n, t, flops, None),
         "cpu_baseline": None,
