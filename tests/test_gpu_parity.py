@@ -245,11 +245,32 @@ def test_overlapped_upload_in_launch_order(cells2k, ref2k, mech, cells):
         L.set_gate(lib, mode=1, cells=cells)
         out, tt, tk = _host_call(lib, d, ident)
         assert tt > 0 and tk > 0, L.last_error(lib)
+        assert L.get_gate(lib)["mode"] == 1, "the call fell back to uploading first"
         _assert_bitexact(out, ref, mech.nvar)
         L.set_gate(lib, mode=0)
         out0, _, _ = _host_call(lib, d, ident)
         for a, b in zip(out, out0):
             assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+    finally:
+        L.set_gate(lib, mode=1, cells=0)
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 200, 2047])
+def test_overlapped_upload_ragged_sizes(cells2k, ref2k, mech, n):
+    """Batches that are not a whole number of 64-cell chunks (a last chunk of
+    one cell, a batch smaller than one chunk) through the overlapped upload:
+    the gate hands out every cell once its chunk has landed, and the results
+    equal the oracle bit for bit."""
+    lib = L.load()
+    d = inputs.subset(cells2k, np.arange(n))
+    ident = np.arange(1, n + 1, dtype=np.int32)
+    print(f"case: overlapped upload, {n} cells, 64-cell chunks", flush=True)
+    try:
+        L.set_gate(lib, mode=1, cells=64)
+        out, tt, _ = _host_call(lib, d, ident)
+        assert tt > 0, L.last_error(lib)
+        assert L.get_gate(lib)["mode"] == 1, "the call fell back to uploading first"
+        _assert_bitexact(out, tuple(x[:n] for x in ref2k[:4]), mech.nvar)
     finally:
         L.set_gate(lib, mode=1, cells=0)
 
