@@ -21,6 +21,12 @@ the sweep; measured HBM traffic already stored there under ``profiles`` is
 kept.  Usage::
 
     python geos-chem-gpu-hybrid_amd/autotune.py [--mech fullchem] [--cells N] [--passes 2]
+    python geos-chem-gpu-hybrid_amd/autotune.py --mech stress500 --workload stress --axes grain,variant
+
+``--workload stress`` tunes on the synthetic stress cells in their own order
+(how the stress batches run, with no previous step to sort by); ``--axes``
+restricts the sweep.  build.py compiles each mechanism's stored launch in as
+the library's default, so callers of the C-ABI get it without Python.
 
 ``--variants`` instead runs one solve of each representative launch shape in
 VARIANTS (after a 1 GiB elementwise copy that calibrates the HBM counters and
@@ -98,6 +104,12 @@ def main(argv=None):
     ap.add_argument("--out", default="")
     ap.add_argument("--variants", action="store_true",
                     help="one solve per shape in VARIANTS (for a profiler), no sweep")
+    ap.add_argument("--workload", default="grid4x5", choices=["grid4x5", "stress"],
+                    help="grid4x5: the 4x5 grid generator in cost order; stress: "
+                         "inputs.stress_cells (seed 46, 150,000 cells unless --cells) in "
+                         "their own order, as the stress batches run")
+    ap.add_argument("--axes", default="",
+                    help="comma-separated launch parameters to sweep (default: all)")
     ap.add_argument("--only", type=int, default=0,
                     help="with --variants: only the shape of this 1-based index")
     args = ap.parse_args(argv)
@@ -117,18 +129,26 @@ def main(argv=None):
         torch.cuda.synchronize()
         del src, dst
         log("dispatch group 0: elementwise copy of 1 GiB (reads 1 GiB, writes 1 GiB)")
-    d = inputs.grid_4x5_cells(mech, seed=1)
-    if args.cells:
-        d = inputs.subset(d, np.arange(args.cells))
+    if args.workload == "stress":
+        d = inputs.stress_cells(mech, args.cells or 150000, seed=46)
+        what = f"inputs.stress_cells seed 46, {d['C'].shape[0]} cells, their own order"
+    else:
+        d = inputs.grid_4x5_cells(mech, seed=1)
+        if args.cells:
+            d = inputs.subset(d, np.arange(args.cells))
+        what = (f"4x5 grid generator (grid_4x5_cells) seed 1, {d['C'].shape[0]} cells, "
+                f"cost order")
     n = d["C"].shape[0]
     b = solver.Batch.from_numpy(d)
     C0 = b.C.clone()
-    r = solver.solve(b, kernel="persistent")
+    r = solver.solve(b, kernel="persistent", mech=args.mech)
     torch.cuda.synchronize()
-    b.Perm = torch.as_tensor(inputs.cost_order(r.ISTAT[:, 2].cpu().numpy()), device=b.C.device)
+    if args.workload != "stress":
+        b.Perm = torch.as_tensor(inputs.cost_order(r.ISTAT[:, 2].cpu().numpy()),
+                                 device=b.C.device)
     ref_C = r.C.cpu().numpy().view(np.int64)
     ref_I = r.ISTAT.cpu().numpy()
-    log(f"autotune {args.mech}: {n} cells, cost order from a first solve")
+    log(f"autotune {args.mech}: {what}")
 
     out = None
 
@@ -136,7 +156,7 @@ def main(argv=None):
         nonlocal out
         L.set_tuning(lib, **t)
         b.C.copy_(C0)
-        out = solver.solve(b, kernel="persistent", out=out)
+        out = solver.solve(b, kernel="persistent", mech=args.mech, out=out)
         torch.cuda.synchronize()
         same = (np.array_equal(out.C.cpu().numpy().view(np.int64), ref_C)
                 and np.array_equal(out.ISTAT.cpu().numpy(), ref_I))
@@ -152,7 +172,7 @@ def main(argv=None):
                 continue
             L.set_tuning(lib, **t)
             b.C.copy_(C0)
-            out = solver.solve(b, kernel=kernel, out=out)
+            out = solver.solve(b, kernel=kernel, mech=args.mech, out=out)
             torch.cuda.synchronize()
             same = (np.array_equal(out.C.cpu().numpy().view(np.int64), ref_C)
                     and np.array_equal(out.ISTAT.cpu().numpy(), ref_I))
@@ -170,9 +190,15 @@ def main(argv=None):
     best_t = measure(cur)
     sweep.append({**cur, "s": best_t})
     log(f"  start {cur}: {best_t:.3f} s")
+    axes = [a for a in args.axes.split(",") if a] or list(SPACE)
+    unknown = set(axes) - set(SPACE)
+    if unknown:
+        raise SystemExit(f"unknown axes {sorted(unknown)}; choose from {list(SPACE)}")
     for p in range(args.passes):
         changed = False
         for key, values in SPACE.items():
+            if key not in axes:
+                continue
             for v in values:
                 if v == cur[key]:
                     continue
@@ -193,8 +219,8 @@ def main(argv=None):
     path = Path(args.out) if args.out else tuning.TUNE_DIR / f"{args.mech}.json"
     data = json.loads(path.read_text()) if path.exists() else {}
     data["launch"] = cur
-    data["autotune"] = {"workload": f"{args.mech} 4x5 grid generator (grid_4x5_cells) seed 1, {n} cells, "
-                                    f"dt 600 s, cost order, persistent kernel",
+    data["autotune"] = {"workload": f"{args.mech} {what}, dt 600 s, persistent kernel",
+                        "axes": axes,
                         "best_kernel_s": best_t, "cells_per_s": n / best_t,
                         "device": torch.cuda.get_device_name(0), "sweep": sweep}
     path.parent.mkdir(parents=True, exist_ok=True)

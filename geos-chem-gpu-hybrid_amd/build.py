@@ -83,6 +83,21 @@ def build_mechanism(mech: str, force: bool = False, jobs: int | None = None) -> 
     return link()
 
 
+TUNE_MACROS = {"block": "GC_TUNE_BLOCK", "blocks_per_cu": "GC_TUNE_BLOCKS_PER_CU",
+               "cells_per_wave": "GC_TUNE_CELLS_PER_WAVE", "grain": "GC_TUNE_GRAIN",
+               "variant": "GC_TUNE_VARIANT"}
+
+
+def tuning_defines(mech: str) -> list[str]:
+    """The autotuned launch stored in tuning/<mech>.json as the library's
+    compiled-in default (-DGC_TUNE_*); none without a stored launch."""
+    p = PKG / "tuning" / f"{mech}.json"
+    if not p.exists():
+        return []
+    launch = json.loads(p.read_text()).get("launch", {})
+    return [f"-D{TUNE_MACROS[k]}={int(v)}" for k, v in sorted(launch.items()) if k in TUNE_MACROS]
+
+
 def plan_mechanism(mech: str, force: bool = False):
     """Generate the mechanism header; return (compile jobs, link function)."""
     sys.path.insert(0, str(PKG / "codegen"))
@@ -112,9 +127,12 @@ def plan_mechanism(mech: str, force: bool = False):
     obj = bdir / "api.o"
     objs.append(obj)
     deps = [CSRC / "gc_api.hip", PKG.parent / "include" / "gcchemgpu.h"] + common
+    tune_file = PKG / "tuning" / f"{mech}.json"
+    if tune_file.exists():
+        deps.append(tune_file)
     if force or _stale(obj, deps):
-        todo.append(([HIPCC, *HIP_FLAGS, f"-I{CSRC}", "-c", str(CSRC / "gc_api.hip"),
-                      "-o", str(obj)], f"{mech}/api.o"))
+        todo.append(([HIPCC, *HIP_FLAGS, *tuning_defines(mech), f"-I{CSRC}", "-c",
+                      str(CSRC / "gc_api.hip"), "-o", str(obj)], f"{mech}/api.o"))
 
     obj = bdir / "hybrid.o"
     objs.append(obj)

@@ -58,12 +58,36 @@ long ws_per_slot() { return variant_info(1).ws_doubles_per_slot; }
 // ---- launch tuning (process-wide) ------------------------------------------
 // blocks_per_cu = 0 means "fill the CU": as many resident work-groups as the
 // variant's occupancy target allows (waves_per_eu * 4 SIMDs per CU).
+// The launch the autotuner stored for this library's mechanism, compiled in by
+// the build (-DGC_TUNE_*); these values where none is stored.
+#ifndef GC_TUNE_BLOCK
+#define GC_TUNE_BLOCK 256
+#endif
+#ifndef GC_TUNE_BLOCKS_PER_CU
+#define GC_TUNE_BLOCKS_PER_CU 0
+#endif
+#ifndef GC_TUNE_CELLS_PER_WAVE
+#define GC_TUNE_CELLS_PER_WAVE 64
+#endif
+#ifndef GC_TUNE_GRAIN
+#define GC_TUNE_GRAIN 64
+#endif
+#ifndef GC_TUNE_VARIANT
+#define GC_TUNE_VARIANT 1
+#endif
+static_assert(GC_TUNE_BLOCK >= 64 && GC_TUNE_BLOCK <= 256 && GC_TUNE_BLOCK % 64 == 0 &&
+                  GC_TUNE_BLOCKS_PER_CU >= 0 && GC_TUNE_BLOCKS_PER_CU <= 32 &&
+                  GC_TUNE_CELLS_PER_WAVE >= 1 && GC_TUNE_CELLS_PER_WAVE <= 64 &&
+                  GC_TUNE_GRAIN >= 1 && GC_TUNE_GRAIN <= 4096 && GC_TUNE_VARIANT >= 1 &&
+                  GC_TUNE_VARIANT <= 3,
+              "compiled-in launch outside the ranges gc_amd_set_tuning accepts");
+
 struct Tuning {
-  int block = 256;
-  int blocks_per_cu = 0;
-  int cells_per_wave = 64;
-  int grain = 64;
-  int variant = 1;
+  int block = GC_TUNE_BLOCK;
+  int blocks_per_cu = GC_TUNE_BLOCKS_PER_CU;
+  int cells_per_wave = GC_TUNE_CELLS_PER_WAVE;
+  int grain = GC_TUNE_GRAIN;
+  int variant = GC_TUNE_VARIANT;
 };
 std::mutex g_tune_mu;
 Tuning g_tune;

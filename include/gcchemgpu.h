@@ -285,7 +285,10 @@ void gc_amd_resident_destroy(GcAmdResident *r);
  *   variant         LU row-segment variant: 1 = 32-entry segments in LDS
  *                   (2 waves per SIMD), 2 = 16-entry segments (4 waves per
  *                   SIMD), 3 = 24-entry segments (3 waves per SIMD)
- * Returns 0, or -1 if a value is out of range (nothing changed). */
+ * The defaults are the launch the autotuner stored for the library's
+ * mechanism, compiled in; GC_AMD_TUNE (environment) overrides them at the
+ * first call.  Returns 0, or -1 if a value is out of range (nothing
+ * changed). */
 int gc_amd_set_tuning(int block, int blocks_per_cu, int cells_per_wave,
                       int grain, int variant);
 void gc_amd_get_tuning(int *block, int *blocks_per_cu, int *cells_per_wave,

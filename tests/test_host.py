@@ -132,3 +132,26 @@ def test_gate_starts_off_under_serializing_tools():
     assert run({**env, "ROCPROF_COUNTER_COLLECTION": "True"}) == "0"
     assert run({**env, "AMD_SERIALIZE_KERNEL": "3"}) == "0"
     assert run({**env, "HIP_LAUNCH_BLOCKING": "0"}) == "1"
+
+
+def test_compiled_in_launch_is_the_stored_tuning():
+    """build.py compiles each mechanism's autotuned launch (tuning/<mech>.json)
+    in as the library's default, so a C-ABI caller that never sets a tuning
+    gets it; a fresh process (no GC_AMD_TUNE) reads it back unchanged."""
+    import json
+    import os
+    import subprocess
+    import sys
+    from conftest import REPO
+    tdir = REPO / "geos-chem-gpu-hybrid_amd" / "tuning"
+    mechs = sorted(p.stem for p in tdir.glob("*.json"))
+    assert "fullchem" in mechs and "stress500" in mechs
+    code = ("import sys, json; sys.path.insert(0, %r); import __graft_entry__ as g; "
+            "g.load_package(); from gcchem_amd import lib as L; "
+            "print(json.dumps({m: L.get_tuning(L.load(m)) for m in %r}))" % (str(REPO), mechs))
+    env = {k: v for k, v in os.environ.items() if k != "GC_AMD_TUNE"}
+    got = json.loads(subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                                    text=True, timeout=300, check=True).stdout.split("\n")[-2])
+    for m in mechs:
+        want = json.loads((tdir / f"{m}.json").read_text())["launch"]
+        assert got[m] == {k: int(v) for k, v in want.items()}, m
