@@ -180,7 +180,7 @@ def main(argv=None):
                          "bit_identical": bool(same)})
             log(f"dispatch group {k}: {label}: {kernel} {t}: {out.t_kernel:.3f} s, "
                 f"bit-identical {same}")
-        print(json.dumps({"workload": f"{args.mech} 4x5 grid seed 1, {n} cells, cost order",
+        print(json.dumps({"workload": f"{args.mech} {what}",
                           "variants": rows}), flush=True)
         return 0 if all(r["bit_identical"] for r in rows) else 1
 
