@@ -150,8 +150,9 @@ def test_compiled_in_launch_is_the_stored_tuning():
             "g.load_package(); from gcchem_amd import lib as L; "
             "print(json.dumps({m: L.get_tuning(L.load(m)) for m in %r}))" % (str(REPO), mechs))
     env = {k: v for k, v in os.environ.items() if k != "GC_AMD_TUNE"}
-    got = json.loads(subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
-                                    text=True, timeout=300, check=True).stdout.split("\n")[-2])
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         timeout=300, check=True).stdout
+    got = json.loads([line for line in out.splitlines() if line.strip()][-1])
     for m in mechs:
         want = json.loads((tdir / f"{m}.json").read_text())["launch"]
         assert got[m] == {k: int(v) for k, v in want.items()}, m
